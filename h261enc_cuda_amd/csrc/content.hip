@@ -1,0 +1,138 @@
+/*
+ * content.hip -- synthetic input frames (content.h) on the device and on
+ * the host.  The device kernel writes whole frame sequences straight into
+ * HBM so that encoding throughput is measured with inputs already resident.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/h261gpu.h"
+#include "content.h"
+#include "h261gpu_internal.h"
+
+namespace h261g {
+
+using content::FrameCtx;
+
+/* frame context of frame t of a stream: displacement sums for the global
+ * mode, scene seed and age for the local mode */
+FrameCtx content_frame_ctx(const h261g_content_params_t &c, int width, int height, int t)
+{
+    FrameCtx f;
+    f.noise_seed = c.seed;
+    f.frame = t;
+    f.sigma = c.sigma;
+    f.local = c.local;
+    f.width = width;
+    f.height = height;
+    f.ox = f.oy = 0;
+    f.age = 0;
+    if (c.local) {
+        const int scene = c.scene_cut > 0 ? t / c.scene_cut : 0;
+        f.tex_seed = content::scene_seed(c.seed, scene);
+        f.age = c.scene_cut > 0 ? t - scene * c.scene_cut : t;
+    } else {
+        f.tex_seed = c.seed;
+        for (int k = 1; k <= t; k++) {
+            f.ox += content::global_step(c.seed, k, 0, c.motion);
+            f.oy += content::global_step(c.seed, k, 1, c.motion);
+        }
+    }
+    return f;
+}
+
+namespace {
+
+/* blockIdx.y = frame of the job; threads cover 4 luma pels or 4 chroma
+ * pel pairs each */
+__global__ __launch_bounds__(256) void content_kernel(const FrameCtx *ctx, uint8_t *out, int width, int height,
+                                                      size_t frame_stride)
+{
+    const FrameCtx f = ctx[blockIdx.y];
+    uint8_t *y = out + frame_stride * blockIdx.y;
+    uint8_t *cb = y + (size_t)width * height;
+    uint8_t *cr = cb + (size_t)width * height / 4;
+    const int n_luma = width * height / 4, n_chroma = width * height / 16;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_luma) {
+        const int px = (4 * i) % width, py = (4 * i) / width;
+        uint32_t w = 0;
+        for (int k = 0; k < 4; k++) w |= (uint32_t)content::luma(f, px + k, py) << (8 * k);
+        *(uint32_t *)(y + 4 * (size_t)i) = w;
+    } else if (i < n_luma + n_chroma) {
+        const int j = i - n_luma, cw = width / 2;
+        const int px = (4 * j) % cw, py = (4 * j) / cw;
+        uint32_t wb = 0, wr = 0;
+        for (int k = 0; k < 4; k++) {
+            uint8_t b, r;
+            content::chroma(f, px + k, py, &b, &r);
+            wb |= (uint32_t)b << (8 * k);
+            wr |= (uint32_t)r << (8 * k);
+        }
+        *(uint32_t *)(cb + 4 * (size_t)j) = wb;
+        *(uint32_t *)(cr + 4 * (size_t)j) = wr;
+    }
+}
+
+}  // namespace
+
+/* frames [first, first + n) of a stream into out (I420, frame after frame) */
+int generate_frames_device(const h261g_content_params_t &c, int width, int height, int first, int n, uint8_t *out,
+                           void *stream)
+{
+    if (n <= 0) return 0;
+    FrameCtx *h = (FrameCtx *)malloc(sizeof(FrameCtx) * n);
+    if (!h) return H261G_ERR_MEMORY;
+    /* incremental displacement sums */
+    FrameCtx f = content_frame_ctx(c, width, height, first);
+    for (int i = 0; i < n; i++) {
+        if (i > 0) {
+            const int t = first + i;
+            if (c.local) {
+                f = content_frame_ctx(c, width, height, t);
+            } else {
+                f.frame = t;
+                f.ox += content::global_step(c.seed, t, 0, c.motion);
+                f.oy += content::global_step(c.seed, t, 1, c.motion);
+            }
+        }
+        h[i] = f;
+    }
+    FrameCtx *d = nullptr;
+    int rc = 0;
+    if (hipMallocAsync((void **)&d, sizeof(FrameCtx) * n, (hipStream_t)stream) != hipSuccess) {
+        free(h);
+        return H261G_ERR_MEMORY;
+    }
+    if (hipMemcpyAsync(d, h, sizeof(FrameCtx) * n, hipMemcpyHostToDevice, (hipStream_t)stream) != hipSuccess)
+        rc = H261G_ERR_DEVICE;
+    const int threads = width * height / 4 + width * height / 16;
+    if (!rc) {
+        hipLaunchKernelGGL(content_kernel, dim3((threads + 255) / 256, n), dim3(256), 0, (hipStream_t)stream, d, out,
+                           width, height, (size_t)width * height * 3 / 2);
+        if (hipGetLastError() != hipSuccess) rc = H261G_ERR_DEVICE;
+    }
+    if (hipFreeAsync(d, (hipStream_t)stream) != hipSuccess && !rc) rc = H261G_ERR_DEVICE;
+    /* the pinned-free host copy must outlive the async upload */
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && !rc) rc = H261G_ERR_DEVICE;
+    free(h);
+    return rc;
+}
+
+}  // namespace h261g
+
+extern "C" int h261g_content_frame_host(const h261g_content_params_t *c, int32_t width, int32_t height, int32_t frame,
+                                        uint8_t *y, uint8_t *cb, uint8_t *cr)
+{
+    if (!c || !y || !cb || !cr || width <= 0 || height <= 0 || width % 16 || height % 16 || frame < 0)
+        return H261G_ERR_ARG;
+    const h261g::content::FrameCtx f = h261g::content_frame_ctx(*c, width, height, frame);
+    for (int py = 0; py < height; py++)
+        for (int px = 0; px < width; px++) y[py * width + px] = h261g::content::luma(f, px, py);
+    for (int py = 0; py < height / 2; py++)
+        for (int px = 0; px < width / 2; px++)
+            h261g::content::chroma(f, px, py, &cb[py * (width / 2) + px], &cr[py * (width / 2) + px]);
+    return H261G_OK;
+}

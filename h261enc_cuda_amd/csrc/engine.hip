@@ -1,0 +1,708 @@
+/*
+ * engine.hip -- the multi-stream encoder (C ABI part 2 of h261gpu.h).
+ *
+ * Every open stream owns, in HBM: its input (one host-fed frame buffer or a
+ * device-generated frame sequence), two reconstruction buffers used as
+ * reference / reconstruction in turn, and its DenoiseState and RcState.
+ * All streams' macroblocks sit in one global MB index space (mb_base per
+ * stream, a map from global MB to stream), so one frame step is six
+ * launches over all streams at once:
+ *
+ *   me_kernel          search + SATD costs + mode decision, cost sum
+ *   rc_begin_kernel    picture cost, rate control, picture quantiser
+ *   mb_kernel          the fused macroblock coder, bit counts
+ *   denoise_update     offsets for the next frame
+ *   frame_bits_kernel  exact frame size (GOB scan), rate-control update
+ *   advance_kernel     swap reference/reconstruction, next input frame
+ *
+ * Nothing in a step depends on the host, so h261g_engine_run captures the
+ * step once in a HIP graph and replays it.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/h261gpu.h"
+#include "h261gpu_internal.h"
+
+namespace h261g {
+
+int generate_frames_device(const h261g_content_params_t &c, int width, int height, int first, int n, uint8_t *out,
+                           void *stream);
+
+namespace {
+
+enum : int32_t {
+    kStreamSeqInput = 4, /* input is a device frame sequence */
+};
+
+/* device-side per-stream input cursor */
+struct SeqState {
+    uint8_t *base;     /* frame 0 of the sequence */
+    int64_t frame_size;
+    int32_t remaining; /* frames left including the current one */
+    int32_t pp;        /* which reconstruction buffer is the reference */
+    uint8_t *recon[2];
+};
+
+__global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    StreamDesc &d = streams[s];
+    if (!(d.flags & kStreamActive)) return;
+    SeqState &q = seq[s];
+    /* the reconstruction becomes the next reference (h261_main.c:247-249) */
+    q.pp ^= 1;
+    const size_t ysz = (size_t)d.width * d.height, csz = ysz / 4;
+    uint8_t *ref = q.recon[q.pp], *rec = q.recon[q.pp ^ 1];
+    d.ref[0] = ref;
+    d.ref[1] = ref + ysz;
+    d.ref[2] = ref + ysz + csz;
+    d.rec[0] = rec;
+    d.rec[1] = rec + ysz;
+    d.rec[2] = rec + ysz + csz;
+    d.frame_num++;
+    d.frame_type = H261_FRAME_TYPE_INTER;
+    if (d.flags & kStreamSeqInput) {
+        q.remaining--;
+        if (q.remaining <= 0) {
+            d.flags &= ~kStreamActive;
+        } else {
+            const uint8_t *cur = q.base + (size_t)q.frame_size * d.frame_num;
+            d.cur[0] = cur;
+            d.cur[1] = cur + ysz;
+            d.cur[2] = cur + ysz + csz;
+        }
+    } else {
+        d.flags &= ~kStreamActive; /* a host-fed stream waits for its next frame */
+    }
+}
+
+bool ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    fprintf(stderr, "h261gpu engine: %s: %s\n", what, hipGetErrorString(e));
+    return false;
+}
+
+struct Slot {
+    bool open = false;
+    int width = 0, height = 0, mb_w = 0, mb_h = 0, n_mb = 0;
+    int mb_base = -1;
+    uint8_t *input = nullptr;    /* host-fed frame buffer (I420) */
+    uint8_t *recon = nullptr;    /* two I420 buffers */
+    uint8_t *seq = nullptr;      /* device frame sequence */
+    int seq_frames = 0;
+    int64_t footprint = 0;
+    bool pending = false;        /* host frame queued for the next step */
+    h261g_stream_params_t params{};
+};
+
+}  // namespace
+
+}  // namespace h261g
+
+using namespace h261g;
+
+struct h261g_engine {
+    int max_streams = 0;
+    int64_t budget = 0, used = 0;
+    int mbs_per_wave = 1;
+    hipStream_t stream = nullptr;
+    std::vector<Slot> slots;
+    /* device arrays indexed by stream slot */
+    StreamDesc *desc = nullptr;
+    SeqState *seq = nullptr;
+    DenoiseState *dn = nullptr;
+    RcState *rc = nullptr;
+    /* global MB space */
+    int mb_cap = 0, mb_used = 0;
+    int32_t *mb_stream = nullptr;
+    MeResult *me = nullptr;
+    MbMode *mode = nullptr;
+    MbSide *side = nullptr;
+    int16_t *levels = nullptr;
+    std::vector<int> free_cif, free_qcif; /* MB ranges of closed streams */
+    h261_macroblock_t *mbstruct = nullptr; /* scratch for read_macroblocks */
+    int mbstruct_cap = 0;
+    /* graph of one step */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_valid = false;
+    /* profiling */
+    bool profile = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double me_ms = 0, mb_ms = 0, other_ms = 0;
+    int64_t launches = 0;
+};
+
+namespace {
+
+int64_t stream_footprint(int w, int h)
+{
+    const int64_t fsz = (int64_t)w * h * 3 / 2, n_mb = (int64_t)(w / 16) * (h / 16);
+    /* input + two reconstructions + per-MB arrays */
+    return 3 * fsz + n_mb * ((int64_t)sizeof(int32_t) + sizeof(MeResult) + sizeof(MbMode) + sizeof(MbSide) + 768);
+}
+
+bool valid_size(int w, int h) { return (w == 352 && h == 288) || (w == 176 && h == 144); }
+
+/* grow the global MB arrays to hold n MBs (contents are per step, so no copy) */
+int ensure_mb_capacity(h261g_engine *e, int n)
+{
+    if (n <= e->mb_cap) return H261G_OK;
+    int cap = e->mb_cap ? e->mb_cap : 1024;
+    while (cap < n) cap *= 2;
+    if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
+    int32_t *ms = nullptr;
+    MeResult *me = nullptr;
+    MbMode *mo = nullptr;
+    MbSide *sd = nullptr;
+    int16_t *lv = nullptr;
+    if (!ok(hipMalloc((void **)&ms, sizeof(int32_t) * cap), "hipMalloc") ||
+        !ok(hipMalloc((void **)&me, sizeof(MeResult) * cap), "hipMalloc") ||
+        !ok(hipMalloc((void **)&mo, sizeof(MbMode) * cap), "hipMalloc") ||
+        !ok(hipMalloc((void **)&sd, sizeof(MbSide) * cap), "hipMalloc") ||
+        !ok(hipMalloc((void **)&lv, sizeof(int16_t) * 384 * (size_t)cap), "hipMalloc")) {
+        hipFree(ms);
+        hipFree(me);
+        hipFree(mo);
+        hipFree(sd);
+        hipFree(lv);
+        return H261G_ERR_MEMORY;
+    }
+    if (e->mb_stream && e->mb_used)
+        (void)hipMemcpy(ms, e->mb_stream, sizeof(int32_t) * e->mb_used, hipMemcpyDeviceToDevice);
+    hipFree(e->mb_stream);
+    hipFree(e->me);
+    hipFree(e->mode);
+    hipFree(e->side);
+    hipFree(e->levels);
+    e->mb_stream = ms;
+    e->me = me;
+    e->mode = mo;
+    e->side = sd;
+    e->levels = lv;
+    e->mb_cap = cap;
+    e->graph_valid = false;
+    return H261G_OK;
+}
+
+void invalidate_graph(h261g_engine *e)
+{
+    e->graph_valid = false;
+}
+
+int stream_count(const h261g_engine *e) { return e->max_streams; }
+
+/* the six launches of one frame step */
+int enqueue_step(h261g_engine *e, bool timed)
+{
+    const int ns = stream_count(e);
+    const int n_mb = e->mb_used;
+    hipStream_t st = e->stream;
+    if (timed) (void)hipEventRecord(e->ev[0], st);
+    MeLaunch ml{};
+    ml.streams = e->desc;
+    ml.mb_stream = e->mb_stream;
+    ml.n_mb = n_mb;
+    ml.result = e->me;
+    ml.mode = e->mode;
+    ml.rc = e->rc;
+    ml.skip_search_on_intra = 1;
+    if (launch_me(ml, st)) return H261G_ERR_DEVICE;
+    if (timed) (void)hipEventRecord(e->ev[1], st);
+    if (launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
+    if (timed) (void)hipEventRecord(e->ev[2], st);
+    MbLaunch bl{};
+    bl.streams = e->desc;
+    bl.mb_stream = e->mb_stream;
+    bl.n_mb = n_mb;
+    bl.mode = e->mode;
+    bl.denoise = e->dn;
+    bl.rc = e->rc;
+    bl.side = e->side;
+    bl.levels = e->levels;
+    bl.mbs_per_wave = e->mbs_per_wave;
+    if (launch_mb(bl, st)) return H261G_ERR_DEVICE;
+    if (timed) (void)hipEventRecord(e->ev[3], st);
+    if (launch_denoise_update(e->dn, e->desc, ns, st)) return H261G_ERR_DEVICE;
+    if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st)) return H261G_ERR_DEVICE;
+    hipLaunchKernelGGL(advance_kernel, dim3((ns + 63) / 64), dim3(64), 0, st, e->desc, e->seq, ns);
+    if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
+    return H261G_OK;
+}
+
+int accumulate_times(h261g_engine *e)
+{
+    if (!ok(hipEventSynchronize(e->ev[3]), "event")) return H261G_ERR_DEVICE;
+    float a = 0, b = 0, c = 0;
+    (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
+    (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
+    (void)hipEventElapsedTime(&c, e->ev[2], e->ev[3]);
+    e->me_ms += a;
+    e->other_ms += b;
+    e->mb_ms += c;
+    e->launches++;
+    return H261G_OK;
+}
+
+Slot *get_slot(h261g_engine *e, int id)
+{
+    if (!e || id < 0 || id >= e->max_streams || !e->slots[id].open) return nullptr;
+    return &e->slots[id];
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t h261g_stream_footprint(int32_t width, int32_t height)
+{
+    if (!valid_size(width, height)) return H261G_ERR_ARG;
+    return stream_footprint(width, height);
+}
+
+int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_budget, int32_t mbs_per_wave)
+{
+    if (!out || max_streams <= 0 || max_streams > kMaxStreams || hbm_budget < 0 || mbs_per_wave < 0)
+        return H261G_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return H261G_ERR_DEVICE;
+    h261g_engine *e = new h261g_engine();
+    e->max_streams = max_streams;
+    e->budget = hbm_budget;
+    e->mbs_per_wave = mbs_per_wave ? mbs_per_wave : 2;
+    e->slots.resize(max_streams);
+    bool good = ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "stream") &&
+                ok(hipMalloc((void **)&e->desc, sizeof(StreamDesc) * max_streams), "hipMalloc") &&
+                ok(hipMalloc((void **)&e->seq, sizeof(SeqState) * max_streams), "hipMalloc") &&
+                ok(hipMalloc((void **)&e->dn, sizeof(DenoiseState) * max_streams), "hipMalloc") &&
+                ok(hipMalloc((void **)&e->rc, sizeof(RcState) * max_streams), "hipMalloc") &&
+                ok(hipMemset(e->desc, 0, sizeof(StreamDesc) * max_streams), "hipMemset") &&
+                ok(hipMemset(e->seq, 0, sizeof(SeqState) * max_streams), "hipMemset");
+    for (int i = 0; good && i < 4; i++) good = ok(hipEventCreate(&e->ev[i]), "event");
+    if (!good) {
+        h261g_engine_destroy(e);
+        return H261G_ERR_DEVICE;
+    }
+    *out = e;
+    return H261G_OK;
+}
+
+int h261g_engine_destroy(h261g_engine_t *e)
+{
+    if (!e) return H261G_ERR_ARG;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto &s : e->slots) {
+        hipFree(s.input);
+        hipFree(s.recon);
+        hipFree(s.seq);
+    }
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    hipFree(e->desc);
+    hipFree(e->seq);
+    hipFree(e->dn);
+    hipFree(e->rc);
+    hipFree(e->mb_stream);
+    hipFree(e->me);
+    hipFree(e->mode);
+    hipFree(e->side);
+    hipFree(e->levels);
+    hipHostFree(e->mbstruct);
+    for (int i = 0; i < 4; i++)
+        if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return H261G_OK;
+}
+
+int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
+{
+    if (!e || !p || !valid_size(p->width, p->height) || p->denoise < 0 || p->init_quant < 1 || p->init_quant > 31 ||
+        p->bitrate < 8 || p->vbv_size < 8 || p->fixed_quant < 0 || p->fixed_quant > 31)
+        return H261G_ERR_ARG;
+    int id = -1;
+    for (int i = 0; i < e->max_streams; i++)
+        if (!e->slots[i].open) {
+            id = i;
+            break;
+        }
+    if (id < 0) return H261G_ERR_FULL;
+    const int64_t fp = stream_footprint(p->width, p->height);
+    if (e->budget && e->used + fp > e->budget) return H261G_ERR_FULL;
+    Slot &s = e->slots[id];
+    const int64_t fsz = (int64_t)p->width * p->height * 3 / 2;
+    s = Slot();
+    s.width = p->width;
+    s.height = p->height;
+    s.mb_w = p->width / 16;
+    s.mb_h = p->height / 16;
+    s.n_mb = s.mb_w * s.mb_h;
+    s.params = *p;
+    if (!ok(hipMalloc((void **)&s.input, fsz), "hipMalloc") || !ok(hipMalloc((void **)&s.recon, 2 * fsz), "hipMalloc")) {
+        hipFree(s.input);
+        hipFree(s.recon);
+        s = Slot();
+        return H261G_ERR_MEMORY;
+    }
+    /* MB range: a free range of a closed stream of the same size, else append */
+    int base = -1;
+    auto &fl = s.n_mb == 396 ? e->free_cif : e->free_qcif;
+    if (!fl.empty()) {
+        base = fl.back();
+        fl.pop_back();
+    } else {
+        base = e->mb_used;
+        int rcode = ensure_mb_capacity(e, e->mb_used + s.n_mb);
+        if (rcode) {
+            hipFree(s.input);
+            hipFree(s.recon);
+            s = Slot();
+            return rcode;
+        }
+        e->mb_used += s.n_mb;
+    }
+    s.mb_base = base;
+    s.open = true;
+    s.footprint = fp;
+    e->used += fp;
+    /* device state: descriptor, reconstruction chain (frame 0 reference is
+     * all zero, h261_main.c:26-28), denoise, rate control */
+    std::vector<int32_t> map(s.n_mb, id);
+    StreamDesc d{};
+    const size_t ysz = (size_t)s.width * s.height, csz = ysz / 4;
+    d.cur[0] = s.input;
+    d.cur[1] = s.input + ysz;
+    d.cur[2] = s.input + ysz + csz;
+    d.ref[0] = s.recon;
+    d.ref[1] = s.recon + ysz;
+    d.ref[2] = s.recon + ysz + csz;
+    d.rec[0] = s.recon + fsz;
+    d.rec[1] = s.recon + fsz + ysz;
+    d.rec[2] = s.recon + fsz + ysz + csz;
+    d.width = s.width;
+    d.height = s.height;
+    d.mb_w = s.mb_w;
+    d.mb_h = s.mb_h;
+    d.mb_base = base;
+    d.frame_type = H261_FRAME_TYPE_INTRA;
+    d.pic_quant = 1;
+    d.flags = kStreamDeviceQuant; /* inactive until it has input */
+    d.frame_num = 0;
+    d.source_format = s.width == 352 ? H261_SOURCE_FORMAT_CIF : H261_SOURCE_FORMAT_QCIF;
+    SeqState q{};
+    q.frame_size = fsz;
+    q.pp = 0;
+    q.recon[0] = s.recon;
+    q.recon[1] = s.recon + fsz;
+    DenoiseState dn{};
+    dn.denoise = p->denoise;
+    RcState rc{};
+    rc.mode = H261_RATECTRL_MODE_CBR;
+    rc.current_quant = p->init_quant; /* h261_init_ratectrl (ratectrl.c:107-127) */
+    rc.prediction_cost[0] = rc.prediction_cost[1] = 100;
+    rc.prediction_size[0] = rc.prediction_size[1] = 1;
+    rc.bitrate = p->bitrate / 8;
+    rc.vbv_size = p->vbv_size / 8;
+    rc.vbv_fill = rc.vbv_size;
+    rc.fixed_quant = p->fixed_quant;
+    bool good = ok(hipMemsetAsync(s.recon, 0, 2 * fsz, e->stream), "hipMemset") &&
+                ok(hipMemcpyAsync(e->mb_stream + base, map.data(), sizeof(int32_t) * s.n_mb, hipMemcpyHostToDevice,
+                                  e->stream),
+                   "hipMemcpy") &&
+                ok(hipMemcpyAsync(e->desc + id, &d, sizeof(d), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
+                ok(hipMemcpyAsync(e->seq + id, &q, sizeof(q), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
+                ok(hipMemcpyAsync(e->dn + id, &dn, sizeof(dn), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
+                ok(hipMemcpyAsync(e->rc + id, &rc, sizeof(rc), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
+                ok(hipStreamSynchronize(e->stream), "sync");
+    if (!good) return H261G_ERR_DEVICE;
+    invalidate_graph(e);
+    return id;
+}
+
+int h261g_stream_close(h261g_engine_t *e, int32_t id)
+{
+    Slot *s = get_slot(e, id);
+    if (!s) return H261G_ERR_ARG;
+    if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
+    StreamDesc d{};
+    if (!ok(hipMemcpy(e->desc + id, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy")) return H261G_ERR_DEVICE;
+    hipFree(s->input);
+    hipFree(s->recon);
+    hipFree(s->seq);
+    e->used -= s->footprint;
+    /* unmap the range (kernels skip MBs mapped to -1) and keep it for reuse */
+    if (!ok(hipMemset(e->mb_stream + s->mb_base, 0xff, sizeof(int32_t) * s->n_mb), "hipMemset")) return H261G_ERR_DEVICE;
+    (s->n_mb == 396 ? e->free_cif : e->free_qcif).push_back(s->mb_base);
+    *s = Slot();
+    invalidate_graph(e);
+    return H261G_OK;
+}
+
+int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_t *y, const uint8_t *cb, const uint8_t *cr)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !y || !cb || !cr || s->seq) return H261G_ERR_ARG;
+    const size_t ysz = (size_t)s->width * s->height, csz = ysz / 4;
+    int32_t flags = kStreamActive | kStreamDeviceQuant;
+    bool good = ok(hipMemcpyAsync(s->input, y, ysz, hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
+                ok(hipMemcpyAsync(s->input + ysz, cb, csz, hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
+                ok(hipMemcpyAsync(s->input + ysz + csz, cr, csz, hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
+                ok(hipMemcpyAsync(&e->desc[id].flags, &flags, sizeof(flags), hipMemcpyHostToDevice, e->stream),
+                   "hipMemcpy") &&
+                ok(hipStreamSynchronize(e->stream), "sync");
+    s->pending = true;
+    return good ? H261G_OK : H261G_ERR_DEVICE;
+}
+
+int h261g_stream_frames_done(h261g_engine_t *e, int32_t id)
+{
+    Slot *s = get_slot(e, id);
+    if (!s) return H261G_ERR_ARG;
+    StreamDesc d;
+    if (!ok(hipStreamSynchronize(e->stream), "sync") ||
+        !ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy"))
+        return H261G_ERR_DEVICE;
+    return d.frame_num;
+}
+
+/* per-MB arrays only carry data within a step, so re-packing needs only the
+ * map and each descriptor's mb_base rewritten */
+int h261g_engine_compact(h261g_engine_t *e)
+{
+    if (!e) return H261G_ERR_ARG;
+    if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
+    std::vector<StreamDesc> d(e->max_streams);
+    if (!ok(hipMemcpy(d.data(), e->desc, sizeof(StreamDesc) * e->max_streams, hipMemcpyDeviceToHost), "hipMemcpy"))
+        return H261G_ERR_DEVICE;
+    std::vector<int32_t> map;
+    for (int i = 0; i < e->max_streams; i++) {
+        Slot &s = e->slots[i];
+        if (!s.open) continue;
+        s.mb_base = (int)map.size();
+        d[i].mb_base = s.mb_base;
+        map.insert(map.end(), s.n_mb, i);
+    }
+    e->mb_used = (int)map.size();
+    e->free_cif.clear();
+    e->free_qcif.clear();
+    bool good = (map.empty() || ok(hipMemcpy(e->mb_stream, map.data(), sizeof(int32_t) * map.size(),
+                                             hipMemcpyHostToDevice), "hipMemcpy")) &&
+                ok(hipMemcpy(e->desc, d.data(), sizeof(StreamDesc) * e->max_streams, hipMemcpyHostToDevice),
+                   "hipMemcpy");
+    invalidate_graph(e);
+    return good ? H261G_OK : H261G_ERR_DEVICE;
+}
+
+int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_content_params_t *c, int32_t n_frames)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !c || n_frames <= 0 || s->seq) return H261G_ERR_ARG;
+    const int64_t fsz = (int64_t)s->width * s->height * 3 / 2;
+    const int64_t bytes = fsz * n_frames;
+    if (e->budget && e->used + bytes > e->budget) return H261G_ERR_FULL;
+    if (!ok(hipMalloc((void **)&s->seq, bytes), "hipMalloc")) return H261G_ERR_MEMORY;
+    int rcode = generate_frames_device(*c, s->width, s->height, 0, n_frames, s->seq, e->stream);
+    if (rcode) {
+        hipFree(s->seq);
+        s->seq = nullptr;
+        return rcode;
+    }
+    e->used += bytes;
+    s->footprint += bytes;
+    s->seq_frames = n_frames;
+    /* point the descriptor at frame 0 and activate */
+    StreamDesc d;
+    if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
+    const size_t ysz = (size_t)s->width * s->height, csz = ysz / 4;
+    const uint8_t *cur = s->seq + (size_t)fsz * d.frame_num;
+    d.cur[0] = cur;
+    d.cur[1] = cur + ysz;
+    d.cur[2] = cur + ysz + csz;
+    d.flags = kStreamActive | kStreamDeviceQuant | kStreamSeqInput;
+    SeqState q;
+    if (!ok(hipMemcpy(&q, e->seq + id, sizeof(q), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
+    q.base = s->seq;
+    q.remaining = n_frames - d.frame_num;
+    if (!ok(hipMemcpy(e->desc + id, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !ok(hipMemcpy(e->seq + id, &q, sizeof(q), hipMemcpyHostToDevice), "hipMemcpy"))
+        return H261G_ERR_DEVICE;
+    return H261G_OK;
+}
+
+int h261g_engine_step(h261g_engine_t *e)
+{
+    if (!e) return H261G_ERR_ARG;
+    int n = 0;
+    for (auto &s : e->slots)
+        if (s.open && (s.pending || s.seq)) n++;
+    int rcode = enqueue_step(e, e->profile);
+    if (rcode) return rcode;
+    if (e->profile && (rcode = accumulate_times(e))) return rcode;
+    for (auto &s : e->slots) s.pending = false;
+    return n;
+}
+
+int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
+{
+    if (!e || n_steps < 0) return H261G_ERR_ARG;
+    if (e->profile) {
+        for (int i = 0; i < n_steps; i++) {
+            int rcode = h261g_engine_step(e);
+            if (rcode < 0) return rcode;
+        }
+        return H261G_OK;
+    }
+    if (!e->graph_valid) {
+        if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+        if (e->graph) (void)hipGraphDestroy(e->graph);
+        e->graph_exec = nullptr;
+        e->graph = nullptr;
+        if (!ok(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "capture")) return H261G_ERR_DEVICE;
+        int rcode = enqueue_step(e, false);
+        hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
+        if (rcode || !ok(ce, "end capture")) return H261G_ERR_DEVICE;
+        if (!ok(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0), "instantiate"))
+            return H261G_ERR_DEVICE;
+        e->graph_valid = true;
+    }
+    for (int i = 0; i < n_steps; i++)
+        if (!ok(hipGraphLaunch(e->graph_exec, e->stream), "graph launch")) return H261G_ERR_DEVICE;
+    for (auto &s : e->slots) s.pending = false;
+    return H261G_OK;
+}
+
+int h261g_engine_sync(h261g_engine_t *e)
+{
+    if (!e) return H261G_ERR_ARG;
+    return ok(hipStreamSynchronize(e->stream), "sync") ? H261G_OK : H261G_ERR_DEVICE;
+}
+
+int h261g_stream_info(h261g_engine_t *e, int32_t id, h261g_frame_info_t *info)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !info) return H261G_ERR_ARG;
+    StreamDesc d;
+    RcState rc;
+    if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        !ok(hipMemcpy(&rc, e->rc + id, sizeof(rc), hipMemcpyDeviceToHost), "hipMemcpy"))
+        return H261G_ERR_DEVICE;
+    /* the descriptor has already advanced to the next frame */
+    info->frame_num = d.frame_num - 1;
+    info->frame_type = d.frame_num - 1 == 0 ? H261_FRAME_TYPE_INTRA : H261_FRAME_TYPE_INTER;
+    info->quant = rc.pic_quant;
+    info->bytes = rc.frame_bytes;
+    return H261G_OK;
+}
+
+int h261g_stream_read_macroblocks(h261g_engine_t *e, int32_t id, h261_macroblock_t *mbs)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !mbs) return H261G_ERR_ARG;
+    std::vector<MbSide> sd(s->n_mb);
+    std::vector<int16_t> lv((size_t)s->n_mb * 384);
+    if (!ok(hipMemcpy(sd.data(), e->side + s->mb_base, sizeof(MbSide) * s->n_mb, hipMemcpyDeviceToHost), "hipMemcpy") ||
+        !ok(hipMemcpy(lv.data(), e->levels + (size_t)s->mb_base * 384, sizeof(int16_t) * 384 * s->n_mb,
+                      hipMemcpyDeviceToHost),
+            "hipMemcpy"))
+        return H261G_ERR_DEVICE;
+    for (int m = 0; m < s->n_mb; m++) {
+        h261_macroblock_t &o = mbs[m];
+        const MbSide &q = sd[m];
+        o.i_macroblock_address = 0;
+        o.i_macroblock_type = q.type;
+        o.i_macroblock_type_flags = q.flags;
+        o.i_macroblock_quant = q.quant;
+        o.rgi_mv[0] = q.mvx;
+        o.rgi_mv[1] = q.mvy;
+        o.i_coded_block_pattern = q.cbp;
+        for (int k = 0; k < 384; k++) o.rgi_tcoeff[k / 64][k % 64] = lv[(size_t)m * 384 + k];
+        o.i_mb_x = (m % s->mb_w) * 16;
+        o.i_mb_y = (m / s->mb_w) * 16;
+    }
+    return H261G_OK;
+}
+
+int h261g_stream_read_recon(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !y || !cb || !cr) return H261G_ERR_ARG;
+    StreamDesc d;
+    if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
+    /* after the step's advance the last reconstruction is the reference */
+    const size_t ysz = (size_t)s->width * s->height, csz = ysz / 4;
+    bool good = ok(hipMemcpy(y, d.ref[0], ysz, hipMemcpyDeviceToHost), "hipMemcpy") &&
+                ok(hipMemcpy(cb, d.ref[1], csz, hipMemcpyDeviceToHost), "hipMemcpy") &&
+                ok(hipMemcpy(cr, d.ref[2], csz, hipMemcpyDeviceToHost), "hipMemcpy");
+    return good ? H261G_OK : H261G_ERR_DEVICE;
+}
+
+int h261g_stream_read_me(h261g_engine_t *e, int32_t id, me_gpu_mb_t *me)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !me) return H261G_ERR_ARG;
+    return ok(hipMemcpy(me, e->me + s->mb_base, sizeof(MeResult) * s->n_mb, hipMemcpyDeviceToHost), "hipMemcpy")
+               ? H261G_OK
+               : H261G_ERR_DEVICE;
+}
+
+int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !y || !cb || !cr) return H261G_ERR_ARG;
+    StreamDesc d;
+    if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
+    const size_t ysz = (size_t)s->width * s->height, csz = ysz / 4;
+    bool good = ok(hipMemcpy(y, d.cur[0], ysz, hipMemcpyDeviceToHost), "hipMemcpy") &&
+                ok(hipMemcpy(cb, d.cur[1], csz, hipMemcpyDeviceToHost), "hipMemcpy") &&
+                ok(hipMemcpy(cr, d.cur[2], csz, hipMemcpyDeviceToHost), "hipMemcpy");
+    return good ? H261G_OK : H261G_ERR_DEVICE;
+}
+
+int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int32_t cap)
+{
+    Slot *s = get_slot(e, id);
+    if (!s || !out || cap < 0) return H261G_ERR_ARG;
+    if (e->mbstruct_cap < s->n_mb) {
+        hipHostFree(e->mbstruct);
+        e->mbstruct = nullptr;
+        e->mbstruct_cap = 0;
+        if (!ok(hipHostMalloc((void **)&e->mbstruct, sizeof(h261_macroblock_t) * s->n_mb, hipHostMallocDefault),
+                "hipHostMalloc"))
+            return H261G_ERR_MEMORY;
+        e->mbstruct_cap = s->n_mb;
+    }
+    int rcode = h261g_stream_read_macroblocks(e, id, e->mbstruct);
+    if (rcode) return rcode;
+    h261g_frame_info_t info;
+    if ((rcode = h261g_stream_info(e, id, &info))) return rcode;
+    return h261g_write_picture(e->mbstruct, s->width, s->height, info.frame_num, info.quant, out, cap);
+}
+
+int h261g_engine_profile(h261g_engine_t *e, int32_t enable)
+{
+    if (!e) return H261G_ERR_ARG;
+    e->profile = enable != 0;
+    e->me_ms = e->mb_ms = e->other_ms = 0;
+    e->launches = 0;
+    return H261G_OK;
+}
+
+int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, double *other_ms, int64_t *launches)
+{
+    if (!e) return H261G_ERR_ARG;
+    if (me_ms) *me_ms = e->me_ms;
+    if (mb_ms) *mb_ms = e->mb_ms;
+    if (other_ms) *other_ms = e->other_ms;
+    if (launches) *launches = e->launches;
+    return H261G_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+/*
+ * h261gpu_internal.h -- definitions shared by the HIP kernels and the host
+ * driver of libh261gpu.so (not part of the public C ABI).
+ *
+ * Device data layout (one stream, plain path; the batched engine keeps
+ * many of these side by side):
+ *   planes      Y w*h, Cb/Cr (w/2)*(h/2) bytes, stride = width, for the
+ *               current frame, the reference and the reconstruction
+ *   per MB      MeResult (costs + mv), MbMode (type/flags/mv/quant), side
+ *               info, int16 levels[6][64] (raster)
+ *   per stream  DenoiseState (offsets + this frame's statistics), RcState
+ */
+#ifndef H261GPU_INTERNAL_H
+#define H261GPU_INTERNAL_H
+
+#include <stdint.h>
+
+#include "../../include/h261gpu_abi.h"
+
+#ifdef __HIPCC__
+#define H261G_HD __host__ __device__
+#else
+#define H261G_HD
+#endif
+
+namespace h261g {
+
+constexpr int kMaxStreams = 65536;
+
+/* per-stream descriptor read by every kernel of a frame step */
+struct StreamDesc {
+    const uint8_t *cur[3]; /* Y, Cb, Cr of the frame being coded */
+    const uint8_t *ref[3]; /* previous reconstruction */
+    uint8_t *rec[3];       /* this frame's reconstruction */
+    int32_t width, height, mb_w, mb_h;
+    int32_t mb_base;    /* first global MB index of this stream */
+    int32_t frame_type; /* H261_FRAME_TYPE_* */
+    int32_t pic_quant;  /* picture quantiser (1..31) when set by the host */
+    int32_t flags;      /* kStream* bits */
+    int32_t frame_num;
+    int32_t source_format;
+    int32_t pad[2];
+};
+
+enum : int32_t {
+    kStreamActive = 1,       /* encode a frame for this stream in this step */
+    kStreamDeviceQuant = 2,  /* picture quantiser comes from RcState.pic_quant */
+};
+
+/* motion estimation result of one MB; same fields as me_gpu_mb_t */
+struct MeResult {
+    int32_t intra_cost, zero_cost, inter_cost, filter_cost;
+    int8_t mvx, mvy;
+    int16_t pad;
+};
+static_assert(sizeof(MeResult) == 20, "MeResult mirrors me_gpu_mb_t");
+
+/* coding mode of one MB (h261_mb_mode_decision_t without cost) */
+struct MbMode {
+    int32_t type, flags, mvx, mvy, quant_adjust, cost;
+};
+
+/* side information of one coded MB, written by the MB coder */
+struct MbSide {
+    int8_t type;  /* 0 intra, 2 inter */
+    int8_t flags; /* MC|FILTER|COEFF */
+    int8_t quant;
+    int8_t cbp;
+    int8_t mvx, mvy;
+    int16_t bits; /* MTYPE(no MQUANT)+CBP+TCOEFF bits, for the frame-size count */
+};
+
+/* DCT denoise state, device resident; same layout as h261_dct_denoise_t */
+struct DenoiseState {
+    int32_t denoise;
+    int32_t intra_count;
+    int32_t intra_noise[64];
+    int32_t intra_offset[64];
+    int32_t inter_count;
+    int32_t inter_noise[64];
+    int32_t inter_offset[64];
+};
+static_assert(sizeof(DenoiseState) == sizeof(h261_dct_denoise_t), "layout");
+
+/* rate control state, device resident; first 9 fields as h261_ratectrl_t */
+struct RcState {
+    int32_t mode, current_quant, picture_cost;
+    int32_t prediction_cost[2], prediction_size[2];
+    int32_t bitrate, vbv_size, vbv_fill;
+    int32_t fixed_quant; /* > 0: picture quantiser override */
+    int32_t pic_quant;   /* quantiser chosen for the current frame */
+    int32_t cost_accum;  /* sum of MD costs of the current frame */
+    int32_t frame_bytes; /* bytes of the last coded frame */
+    int32_t frame_bits;
+    int32_t pad[3];
+};
+
+/* ---- launchers (implemented in the .hip files) ---- */
+
+/* one wave per MB: two-stage SAD search, the three SATD costs, and (when
+ * mode != nullptr) the reference mode decision for the MB.
+ * Launch covers n_mb global MBs; mb_stream maps a global MB to its stream. */
+struct MeLaunch {
+    const StreamDesc *streams;
+    const int32_t *mb_stream;
+    int32_t n_mb;
+    MeResult *result;   /* [n_mb] */
+    int8_t *mv_start;   /* optional [2*n_mb]: refined vector (me_gpu_t starting vector) */
+    MbMode *mode;       /* optional [n_mb]: device mode decision */
+    RcState *rc;        /* optional per-stream: cost accumulation */
+    int32_t skip_search_on_intra; /* 1: intra frames only compute the intra cost */
+};
+int launch_me(const MeLaunch &p, void *stream);
+
+/* one wave per MB: MC/loop filter, DCT, denoise, quant, dequant, IDCT,
+ * reconstruction; statistics for the denoise update; bit counts. */
+struct MbLaunch {
+    const StreamDesc *streams;
+    const int32_t *mb_stream;
+    int32_t n_mb;
+    const MbMode *mode;           /* [n_mb] */
+    DenoiseState *denoise;        /* per stream */
+    const RcState *rc;            /* per stream, used with kStreamDeviceQuant */
+    MbSide *side;                 /* optional [n_mb] */
+    int16_t *levels;              /* optional [n_mb][6][64] */
+    h261_macroblock_t *mb_struct; /* optional [n_mb] legacy layout */
+    int32_t mbs_per_wave;         /* MBs coded by one wave in sequence */
+};
+int launch_mb(const MbLaunch &p, void *stream);
+
+/* per stream, 64 lanes: new denoise offsets from this frame's statistics,
+ * statistics reset (src/mbgpu_kernels.cuh:478-503) */
+int launch_denoise_update(DenoiseState *dn, const StreamDesc *streams, int n_streams, void *stream);
+
+/* per stream, one thread: picture cost, rate-control picture init */
+int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int num_mb_unused,
+                    void *stream);
+/* per stream, one wave: GOB scan of the MBA / MVD / MQUANT bits, frame
+ * size, rate-control update */
+int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, const MbSide *side,
+                      int32_t *frame_bytes_out, void *stream);
+
+/* synthetic content (see content.hip) */
+struct ContentParams {
+    uint64_t seed;
+    int32_t width, height;
+    int32_t off_x, off_y; /* global texture offset of this frame */
+    int32_t frame;        /* frame index (noise) */
+    int32_t sigma;        /* noise standard deviation, integer */
+    const int16_t *mb_offset; /* optional per-MB (dx,dy) offsets added to off_x/off_y */
+};
+int launch_content(const ContentParams *params_dev, const int32_t *n_params, uint8_t *const *planes,
+                   int n_streams, int max_w, int max_h, void *stream);
+
+}  // namespace h261g
+
+#endif

@@ -1,0 +1,497 @@
+/*
+ * mb_kernel.hip -- the fused macroblock coder for gfx950.
+ *
+ * The reference runs 12 kernels per frame with a thread per 8x8 block and a
+ * global-memory round trip through an interleaved Int32 coefficient buffer
+ * between each (src/gpu_device.cu:592-815).  Here one wave64 codes a whole
+ * macroblock (6 blocks, 384 samples) in registers and LDS:
+ *
+ *   lanes 0-47 = (block b, row r):   MC / loop filter, residual, DCT rows
+ *   LDS transpose
+ *   lanes 0-47 = (block b, col c):   DCT columns, denoise, quantiser, CBP,
+ *                                    dequantiser, IDCT columns
+ *   LDS transpose
+ *   lanes 0-47 = (block b, row r):   IDCT rows, reconstruction
+ *
+ * Arithmetic follows the reference op for op (compiled with
+ * -ffp-contract=off and '#pragma clang fp contract(off)'): products with an
+ * unsuffixed literal are double multiplies rounded to float; the forward
+ * DCT rounds with floorf((float)(x + 0.49999)), the IDCT with
+ * floor((double)x + 0.49999) (src/mbgpu_kernels.cuh:337-442, 827-931).
+ * The denoise uses the previous frame's offsets and accumulates this
+ * frame's |coefficient| statistics (LDS atomics, one global flush per
+ * workgroup); the CBP is the OR of this frame's coded blocks, the intended
+ * result of the reference's racy reset/atomicOr (mbgpu:676-721).
+ */
+#include <hip/hip_runtime.h>
+
+#include "h261_vlc.h"
+#include "h261gpu_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace h261g {
+
+namespace {
+
+/* float tables initialised from double literals (mbgpu:335, 823) */
+__constant__ float c_fwd_post[8] = {1.0, 0.720960, 0.765367, 0.850430, 1.0, 1.272759, 1.847759, 3.624510};
+__constant__ float c_inv_pre[8] = {1.0, 1.387040, 1.306563, 1.175876, 1.0, 0.785695, 0.541196, 0.275899};
+
+/* f_inv_quant = 1.0 / (float)(2q) as a double quotient rounded to float
+ * (mbgpu:685); folded exactly at compile time */
+#define H261G_INVQ(q) ((float)(1.0 / (double)(float)(2 * (q))))
+__constant__ float c_inv_quant[32] = {
+    0.0f,           H261G_INVQ(1),  H261G_INVQ(2),  H261G_INVQ(3),  H261G_INVQ(4),  H261G_INVQ(5),
+    H261G_INVQ(6),  H261G_INVQ(7),  H261G_INVQ(8),  H261G_INVQ(9),  H261G_INVQ(10), H261G_INVQ(11),
+    H261G_INVQ(12), H261G_INVQ(13), H261G_INVQ(14), H261G_INVQ(15), H261G_INVQ(16), H261G_INVQ(17),
+    H261G_INVQ(18), H261G_INVQ(19), H261G_INVQ(20), H261G_INVQ(21), H261G_INVQ(22), H261G_INVQ(23),
+    H261G_INVQ(24), H261G_INVQ(25), H261G_INVQ(26), H261G_INVQ(27), H261G_INVQ(28), H261G_INVQ(29),
+    H261G_INVQ(30), H261G_INVQ(31)};
+#undef H261G_INVQ
+
+__constant__ uint8_t c_zigzag[64] = H261G_ZIGZAG;
+__constant__ Vlc c_tcoeff[kTcoeffRuns][kTcoeffLevels] = H261G_TCOEFF_TABLE;
+__constant__ Vlc c_cbp[64] = H261G_CBP_TABLE;
+
+constexpr int kCbpBit[6] = {32, 16, 8, 4, 2, 1};
+constexpr int kBlkPitch = 72; /* floats per block in the transpose buffer */
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* the 1-D AAN forward transform shared by the row and column passes
+ * (mbgpu:350-387 and 399-437); o[k] gets output k before the postscale */
+struct Fwd8 {
+    float o[8];
+};
+
+__device__ __forceinline__ void fdct_core(float tmp0, float tmp1, float tmp2, float tmp3, float tmp4,
+                                          float tmp5, float tmp6, float tmp7, float *o)
+{
+    float tmp10 = tmp0 + tmp3, tmp13 = tmp0 - tmp3, tmp11 = tmp1 + tmp2, tmp12 = tmp1 - tmp2;
+    o[0] = tmp10 + tmp11;
+    o[4] = tmp10 - tmp11;
+    float z1 = tmp12 + tmp13;
+    z1 = (float)((double)z1 * 0.707107);
+    o[2] = tmp13 + z1;
+    o[6] = tmp13 - z1;
+    tmp4 += tmp5;
+    tmp5 += tmp6;
+    tmp6 += tmp7;
+    float z5 = (float)((double)(tmp4 - tmp6) * 0.382683);
+    float z2 = (float)((double)tmp4 * 0.541196 + (double)z5);
+    float z4 = (float)((double)tmp6 * 1.306563 + (double)z5);
+    float z3 = (float)((double)tmp5 * 0.707107);
+    float z11 = tmp7 + z3, z13 = tmp7 - z3;
+    o[5] = z13 + z2;
+    o[3] = z13 - z2;
+    o[1] = z11 + z4;
+    o[7] = z11 - z4;
+}
+
+/* the 1-D AAN inverse transform of the column and row passes
+ * (mbgpu:851-887 and 892-919); returns outputs 0..7 before rounding */
+__device__ __forceinline__ void idct_core(const float *t, float *o)
+{
+    float tmp13 = t[2] + t[6];
+    float tmp12 = t[2] - t[6];
+    tmp12 = (float)((double)tmp12 * 1.414214);
+    tmp12 -= tmp13;
+    float tmp10 = t[0] + t[4], tmp11 = t[0] - t[4];
+    float tmp0 = tmp10 + tmp13, tmp1 = tmp11 + tmp12, tmp2 = tmp11 - tmp12, tmp3 = tmp10 - tmp13;
+    float z11 = t[1] + t[7], z12 = t[1] - t[7], z13 = t[5] + t[3], z10 = t[5] - t[3];
+    tmp13 = z11 + z13;
+    tmp11 = (float)((double)(z11 - z13) * 1.414214);
+    float z5 = (float)((double)(z12 + z10) * 1.84776);
+    tmp10 = (float)((double)z12 * 1.082392 - (double)z5);
+    tmp12 = (float)((double)z10 * -2.613126 + (double)z5);
+    tmp12 -= tmp13;
+    tmp11 -= tmp12;
+    tmp10 += tmp11;
+    o[0] = tmp0 + tmp13;
+    o[1] = tmp1 + tmp12;
+    o[2] = tmp2 + tmp11;
+    o[3] = tmp3 - tmp10;
+    o[4] = tmp3 + tmp10;
+    o[5] = tmp2 - tmp11;
+    o[6] = tmp1 - tmp12;
+    o[7] = tmp0 - tmp13;
+}
+
+__device__ __forceinline__ int tcoeff_bits(int run, int level)
+{
+    int a = level < 0 ? -level : level;
+    if (run < kTcoeffRuns && a < kTcoeffLevels) {
+        int len = c_tcoeff[run][a].len;
+        if (len) return len + 1;
+    }
+    return kEscapeBits;
+}
+
+__global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
+{
+    __shared__ float tf[6 * kBlkPitch];
+    __shared__ int32_t lv[6 * 64];
+    __shared__ int32_t stat[2][64];
+    __shared__ int32_t scount[2];
+    __shared__ int32_t sblk[8];
+
+    const int lane = threadIdx.x;
+    const int b = lane >> 3; /* block 0-5 (6, 7 idle) */
+    const int rc = lane & 7; /* row in phases 1/3, column in phase 2 */
+    const bool act = b < 6;
+
+    stat[0][lane] = 0;
+    stat[1][lane] = 0;
+    if (lane < 2) scount[lane] = 0;
+    int flush_stream = -1;
+
+    for (int it = 0; it < p.mbs_per_wave; it++) {
+        const int g = blockIdx.x * p.mbs_per_wave + it;
+        if (g >= p.n_mb) break;
+        const int s = p.mb_stream[g];
+        if (s < 0) continue; /* unmapped range of a closed stream */
+        const StreamDesc &d = p.streams[s];
+        if (!(d.flags & kStreamActive)) continue;
+        if (flush_stream >= 0 && s != flush_stream) {
+            /* a workgroup's MBs span two streams: flush the first one's statistics */
+            __syncthreads();
+            DenoiseState *dn = p.denoise + flush_stream;
+            if (stat[0][lane]) atomicAdd(&dn->intra_noise[lane], stat[0][lane]);
+            if (stat[1][lane]) atomicAdd(&dn->inter_noise[lane], stat[1][lane]);
+            if (lane == 0 && scount[0]) atomicAdd(&dn->intra_count, scount[0]);
+            if (lane == 1 && scount[1]) atomicAdd(&dn->inter_count, scount[1]);
+            __syncthreads();
+            stat[0][lane] = 0;
+            stat[1][lane] = 0;
+            if (lane < 2) scount[lane] = 0;
+            __syncthreads();
+        }
+        flush_stream = s;
+        const int m = g - d.mb_base;
+        const MbMode md = p.mode[g];
+        const int W = d.width;
+        const int bx = (m % d.mb_w) * 16, by = (m / d.mb_w) * 16;
+        const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[s].pic_quant : d.pic_quant;
+        const int quant = clampi(picq + md.quant_adjust, 1, 31); /* gpu_device.cu:625-627 */
+        const bool intra = md.type == H261_MB_TYPE_INTRA;
+        const bool filter = (md.flags & H261_MB_FILTER) != 0;
+        const int t = intra ? 0 : 1;
+
+        /* ---- phase 1: lane (b, r) ---- */
+        const int plane = b < 4 ? 0 : b - 3;
+        int px, py, stride, mvx, mvy;
+        if (b < 4) {
+            px = bx + (b & 1) * 8;
+            py = by + ((b >> 1) & 1) * 8;
+            stride = W;
+            mvx = md.mvx;
+            mvy = md.mvy;
+        } else { /* chroma vector = luma vector / 2, truncated (mbgpu:266-269) */
+            px = bx >> 1;
+            py = by >> 1;
+            stride = W >> 1;
+            mvx = md.mvx / 2;
+            mvy = md.mvy / 2;
+        }
+        int pred[8], res[8];
+        if (act) {
+            const uint8_t *cp = d.cur[plane] + (size_t)(py + rc) * stride + px;
+            uint2 cw = *(const uint2 *)cp;
+            int cpel[8];
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                cpel[x] = (cw.x >> (8 * x)) & 0xff;
+                cpel[x + 4] = (cw.y >> (8 * x)) & 0xff;
+            }
+            if (intra) {
+#pragma unroll
+                for (int x = 0; x < 8; x++) pred[x] = 0;
+            } else {
+                const uint8_t *rp = d.ref[plane] + (size_t)(py + mvy) * stride + px + mvx;
+                if (!filter) {
+#pragma unroll
+                    for (int x = 0; x < 8; x++) pred[x] = rp[rc * stride + x];
+                } else {
+                    /* H.261 loop filter, separable [1 2 1]: 1-D at the
+                     * block edges, corners unfiltered (mbgpu:79-150) */
+                    int h[3][8];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        int rr = clampi(rc + k - 1, 0, 7);
+                        const uint8_t *q = rp + rr * stride;
+                        int a[8];
+#pragma unroll
+                        for (int x = 0; x < 8; x++) a[x] = q[x];
+                        h[k][0] = 4 * a[0];
+#pragma unroll
+                        for (int x = 1; x < 7; x++) h[k][x] = a[x - 1] + 2 * a[x] + a[x + 1];
+                        h[k][7] = 4 * a[7];
+                    }
+                    if (rc == 0 || rc == 7) {
+#pragma unroll
+                        for (int x = 0; x < 8; x++) pred[x] = (h[1][x] + 2) >> 2;
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < 8; x++) pred[x] = (h[0][x] + 2 * h[1][x] + h[2][x] + 8) >> 4;
+                    }
+                }
+            }
+#pragma unroll
+            for (int x = 0; x < 8; x++) res[x] = cpel[x] - pred[x];
+
+            /* forward DCT, row pass */
+            float o[8];
+            fdct_core((float)(res[0] + res[7]), (float)(res[1] + res[6]), (float)(res[2] + res[5]),
+                      (float)(res[3] + res[4]), (float)(res[3] - res[4]), (float)(res[2] - res[5]),
+                      (float)(res[1] - res[6]), (float)(res[0] - res[7]), o);
+#pragma unroll
+            for (int k = 0; k < 8; k++) tf[b * kBlkPitch + rc * 8 + k] = o[k];
+        }
+        __syncthreads();
+
+        /* ---- phase 2: lane (b, c) ---- */
+        int coef[8];
+        int lvl[8];
+        int blk_accum = 0;
+        if (act) {
+            const int c = rc;
+            float v[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) v[r] = tf[b * kBlkPitch + r * 8 + c];
+            float cs = c_fwd_post[c];
+            cs = (float)((double)cs * 0.125);
+            float o[8];
+            fdct_core(v[0] + v[7], v[1] + v[6], v[2] + v[5], v[3] + v[4], v[3] - v[4], v[2] - v[5], v[1] - v[6],
+                      v[0] - v[7], o);
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                coef[k] = (int)floorf((float)((double)(cs * c_fwd_post[k] * o[k]) + 0.49999));
+
+            /* denoise with the previous frame's offsets, staged as UInt16
+             * (mbgpu:525-528, 586-604) */
+            const DenoiseState &dn = p.denoise[s];
+            const int32_t *offs = intra ? dn.intra_offset : dn.inter_offset;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                int i = c + 8 * r;
+                int off = (int)(uint16_t)offs[i];
+                int cc = coef[r];
+                int a = cc < 0 ? -cc : cc;
+                atomicAdd(&stat[t][i], a);
+                a = a - off;
+                a = a > 0 ? a : 0;
+                coef[r] = cc < 0 ? -a : a;
+            }
+
+            /* forward quantiser (mbgpu:685-716) */
+            const float inv = c_inv_quant[quant];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                int i = c + 8 * r;
+                int l;
+                if (intra && i == 0) {
+                    l = coef[r] / 8;
+                    l = clampi(l, 1, 254);
+                    if (l == 128) l = 255;
+                    blk_accum += 100;
+                } else {
+                    float f = (float)coef[r] * inv;
+                    l = clampi((int)f, -127, 127);
+                    int al = l < 0 ? -l : l;
+                    blk_accum += al > 1 ? 100 : al;
+                }
+                lvl[r] = l;
+                lv[b * 64 + i] = l;
+            }
+        }
+        /* sum the 8 columns of the block */
+        blk_accum += __shfl_xor(blk_accum, 1, 64);
+        blk_accum += __shfl_xor(blk_accum, 2, 64);
+        blk_accum += __shfl_xor(blk_accum, 4, 64);
+        const bool coded = act && blk_accum > 2;
+        /* cbp of the MB: ballot of the lanes c == 0 of coded blocks */
+        const uint64_t coded_mask = __ballot(coded && rc == 0);
+        int cbp = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+            if ((coded_mask >> (8 * k)) & 1) cbp |= kCbpBit[k];
+
+        if (act) {
+            const int c = rc;
+            /* dequantiser (mbgpu:763-814); an uncoded block is all zero */
+            const int qadd = coded ? ((quant - 1) | 1) : 0, qmul = coded ? quant * 2 : 0;
+            int deq[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                int i = c + 8 * r;
+                int l = lvl[r];
+                if (intra && i == 0) {
+                    deq[r] = l == 255 ? 1024 : l * 8;
+                } else {
+                    int x = l * qmul;
+                    if (x > 0)
+                        x += qadd;
+                    else if (x < 0)
+                        x -= qadd;
+                    deq[r] = clampi(x, -2048, 2047);
+                }
+            }
+            /* IDCT prescale and column pass (mbgpu:836-888) */
+            float tv[8], o[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                float cs = (float)((double)c_inv_pre[r] * 0.125);
+                tv[r] = ((float)deq[r] * c_inv_pre[c]) * cs;
+            }
+            idct_core(tv, o);
+            __builtin_amdgcn_s_barrier(); /* all phase-2 reads of tf are done */
+#pragma unroll
+            for (int r = 0; r < 8; r++) tf[b * kBlkPitch + r * 8 + c] = o[r];
+        } else {
+            __builtin_amdgcn_s_barrier();
+        }
+        __syncthreads();
+
+        /* ---- phase 3: lane (b, r) ---- */
+        if (act) {
+            float v[8], o[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = tf[b * kBlkPitch + rc * 8 + k];
+            idct_core(v, o);
+            uint32_t w0 = 0, w1 = 0;
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                int rres = clampi((int)floor((double)o[x] + 0.49999), -256, 255);
+                int val = clampi(pred[x] + rres, 0, 255);
+                if (x < 4)
+                    w0 |= (uint32_t)val << (8 * x);
+                else
+                    w1 |= (uint32_t)val << (8 * (x - 4));
+            }
+            *(uint2 *)(d.rec[plane] + (size_t)(py + rc) * stride + px) = make_uint2(w0, w1);
+        }
+
+        /* ---- bit count of the MB's CBP + TCOEFF layer ---- */
+        int bits = 0;
+        const bool written_any = intra || cbp;
+        if (p.side && written_any) {
+            /* lane k = transmission position k of each block in turn */
+            const int zz = c_zigzag[lane];
+            for (int k = 0; k < 6; k++) {
+                const bool blk_written = intra || (cbp & kCbpBit[k]);
+                if (!blk_written) continue;
+                int l = lv[k * 64 + zz];
+                uint64_t nz = __ballot(l != 0);
+                if (l != 0) {
+                    uint64_t below = lane ? (nz & ((~0ull) >> (64 - lane))) : 0ull;
+                    int prev = below ? 63 - __builtin_clzll(below) : -1;
+                    int run = lane - prev - 1;
+                    if (lane == 0) {
+                        if (intra)
+                            bits += 8; /* DC: 8-bit fixed length */
+                        else if (l == 1 || l == -1)
+                            bits += 2; /* first coefficient '1s' */
+                        else
+                            bits += tcoeff_bits(0, l);
+                    } else {
+                        bits += tcoeff_bits(run, l);
+                    }
+                }
+                if (lane == 0) bits += 2; /* EOB */
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) bits += __shfl_xor(bits, off, 64);
+            if (!intra) bits += c_cbp[cbp].len;
+        }
+        __syncthreads();
+
+        /* ---- outputs ---- */
+        const int flags = md.flags | (cbp ? H261_MB_COEFF : 0);
+        if (p.levels) {
+            int16_t *out = p.levels + (size_t)g * 384;
+#pragma unroll
+            for (int k = 0; k < 6; k++) out[k * 64 + lane] = (int16_t)lv[k * 64 + lane];
+        }
+        if (p.mb_struct) {
+            h261_macroblock_t *o = p.mb_struct + g;
+#pragma unroll
+            for (int k = 0; k < 6; k++) o->rgi_tcoeff[k][lane] = lv[k * 64 + lane];
+            if (lane == 0) {
+                o->i_macroblock_address = 0;
+                o->i_macroblock_type = md.type;
+                o->i_macroblock_type_flags = flags;
+                o->i_macroblock_quant = quant;
+                o->rgi_mv[0] = md.mvx;
+                o->rgi_mv[1] = md.mvy;
+                o->i_coded_block_pattern = cbp;
+                o->i_mb_x = bx;
+                o->i_mb_y = by;
+            }
+        }
+        if (p.side && lane == 0) {
+            MbSide sd;
+            sd.type = (int8_t)md.type;
+            sd.flags = (int8_t)flags;
+            sd.quant = (int8_t)quant;
+            sd.cbp = (int8_t)cbp;
+            sd.mvx = (int8_t)md.mvx;
+            sd.mvy = (int8_t)md.mvy;
+            sd.bits = (int16_t)bits;
+            p.side[g] = sd;
+        }
+        if (lane == 0) atomicAdd(&scount[t], 6);
+        __syncthreads();
+    }
+    (void)sblk;
+    __syncthreads();
+    if (flush_stream >= 0) {
+        DenoiseState *dn = p.denoise + flush_stream;
+        if (stat[0][lane]) atomicAdd(&dn->intra_noise[lane], stat[0][lane]);
+        if (stat[1][lane]) atomicAdd(&dn->inter_noise[lane], stat[1][lane]);
+        if (lane == 0 && scount[0]) atomicAdd(&dn->intra_count, scount[0]);
+        if (lane == 1 && scount[1]) atomicAdd(&dn->inter_count, scount[1]);
+    }
+}
+
+/* macroblocks_update_dct_denoise (src/mbgpu_kernels.cuh:478-503) */
+__global__ __launch_bounds__(64) void denoise_update_kernel(DenoiseState *dn_all, const StreamDesc *streams)
+{
+    const int s = blockIdx.x;
+    if (!(streams[s].flags & kStreamActive)) return;
+    DenoiseState *dn = dn_all + s;
+    const int i = threadIdx.x;
+    const int ni = dn->inter_noise[i], na = dn->intra_noise[i];
+    const int ci = dn->inter_count, ca = dn->intra_count;
+    dn->inter_offset[i] = (ci * dn->denoise + ni / 2) / (ni + 1);
+    dn->intra_offset[i] = (ca * dn->denoise + na / 2) / (na + 1);
+    dn->inter_noise[i] = 0;
+    dn->intra_noise[i] = 0;
+    __syncthreads();
+    if (i == 0) {
+        dn->inter_count = 0;
+        dn->intra_count = 0;
+    }
+}
+
+}  // namespace
+
+int launch_mb(const MbLaunch &p, void *stream)
+{
+    if (p.n_mb <= 0) return 0;
+    const int per = p.mbs_per_wave > 0 ? p.mbs_per_wave : 1;
+    MbLaunch q = p;
+    q.mbs_per_wave = per;
+    const int grid = (p.n_mb + per - 1) / per;
+    hipLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, q);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_denoise_update(DenoiseState *dn, const StreamDesc *streams, int n_streams, void *stream)
+{
+    if (n_streams <= 0) return 0;
+    hipLaunchKernelGGL(denoise_update_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, dn, streams);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace h261g
