@@ -1,0 +1,159 @@
+"""Serving: encode requests admitted as they arrive, coded together.
+
+A request is a whole synthetic sequence (QCIF or CIF, its length and
+bitrate target).  The loop admits every arrived request whose device
+footprint (stream state + its frames in HBM) fits the HBM budget, then runs
+one frame step over all admitted streams -- requests that arrive while
+others are coding join the next step (continuous batching); finished
+streams are closed and their memory and macroblock range reused.
+
+Per-frame latency = time from the frame becoming eligible (its request
+admitted and its previous frame done) to the end of the step that coded it
+(levels, side information and the exact frame size on the device).
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .encoder import Engine
+
+
+@dataclasses.dataclass
+class Request:
+    arrival: float          # seconds after the start
+    width: int
+    height: int
+    frames: int
+    bitrate: int
+    seed: int
+    sigma: int = 4
+    motion: int = 0
+    local: bool = True
+    scene_cut: int = 50
+
+
+def poisson_requests(rate: float, duration: float, seed: int = 4, cif_share: float = 0.6,
+                     median_frames: int = 120, sigma: float = 0.8, cap: int = 1800,
+                     rate_lo: int = 64000, rate_hi: int = 1024000) -> List[Request]:
+    """Poisson arrivals; 60% CIF / 40% QCIF; lognormal length (median 120,
+    sigma 0.8, cap 1800); bitrate uniform in [64, 1024] kbit/s; content with
+    per-macroblock local motion and scene cuts every 50 frames."""
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            return out
+        cif = rng.random() < cif_share
+        n = int(min(cap, max(1, round(median_frames * np.exp(sigma * rng.standard_normal())))))
+        out.append(Request(arrival=t, width=352 if cif else 176, height=288 if cif else 144, frames=n,
+                           bitrate=int(rng.integers(rate_lo, rate_hi + 1)), seed=int(rng.integers(1, 2 ** 62))))
+
+
+@dataclasses.dataclass
+class ServingStats:
+    requests: int
+    frames: int
+    cif_equivalent_frames: float
+    seconds: float
+    frame_latency_ms_p50: float
+    frame_latency_ms_p99: float
+    request_latency_ms_p50: float
+    request_latency_ms_p99: float
+    max_concurrent: int
+    steps: int
+
+    @property
+    def frames_per_s(self) -> float:
+        return self.frames / self.seconds
+
+    @property
+    def cif_frames_per_s(self) -> float:
+        return self.cif_equivalent_frames / self.seconds
+
+
+class ServingLoop:
+    def __init__(self, hbm_budget: int = 64 << 30, max_streams: int = 4096, mbs_per_wave: int = 0):
+        self.eng = Engine(max_streams=max_streams, hbm_budget=hbm_budget, mbs_per_wave=mbs_per_wave)
+
+    def run(self, requests: List[Request], collect: bool = False, realtime: bool = True):
+        """Serves the requests.  collect=True also returns every request's
+        per-frame bitstreams (slow: the host writer runs after each step)."""
+        eng = self.eng
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting: collections.deque = collections.deque()
+        active: Dict[int, dict] = {}
+        frame_lat: List[float] = []
+        req_lat: List[float] = []
+        outputs: Dict[int, List[bytes]] = {}
+        index = {id(r): i for i, r in enumerate(requests)}
+        t_start = time.perf_counter()
+        frames = 0
+        mbs = 0
+        steps = 0
+        max_conc = 0
+        while pending or waiting or active:
+            now = time.perf_counter() - t_start
+            while pending and (pending[0].arrival <= now or not realtime):
+                waiting.append(pending.popleft())
+            while waiting:
+                r = waiting[0]
+                try:
+                    sid = eng.open(r.width, r.height, denoise=5, init_quant=10, bitrate=r.bitrate,
+                                   vbv_size=r.bitrate)
+                except RuntimeError:
+                    break  # no slot / budget: admit when memory frees up
+                try:
+                    eng.generate_input(sid, r.seed, r.frames, sigma=r.sigma, motion=r.motion, local=r.local,
+                                       scene_cut=r.scene_cut)
+                except RuntimeError:
+                    eng.close(sid)
+                    break
+                waiting.popleft()
+                t_adm = time.perf_counter() - t_start
+                active[sid] = dict(req=r, done=0, eligible=t_adm, arrival=r.arrival if realtime else 0.0)
+                if collect:
+                    outputs[index[id(r)]] = []
+            if not active:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t_start), 0.01)))
+                continue
+            max_conc = max(max_conc, len(active))
+            eng.step()
+            eng.sync()
+            steps += 1
+            t_end = time.perf_counter() - t_start
+            finished = []
+            for sid, a in active.items():
+                r = a["req"]
+                frame_lat.append((t_end - a["eligible"]) * 1e3)
+                a["eligible"] = t_end
+                a["done"] += 1
+                frames += 1
+                mbs += (r.width // 16) * (r.height // 16)
+                if collect:
+                    outputs[index[id(r)]].append(eng.bitstream(sid))
+                if a["done"] >= r.frames:
+                    finished.append(sid)
+                    req_lat.append((t_end - a["arrival"]) * 1e3)
+            for sid in finished:
+                eng.close(sid)
+                del active[sid]
+        seconds = time.perf_counter() - t_start
+        fl = np.array(frame_lat) if frame_lat else np.zeros(1)
+        rl = np.array(req_lat) if req_lat else np.zeros(1)
+        stats = ServingStats(requests=len(requests), frames=frames, cif_equivalent_frames=mbs / 396.0,
+                             seconds=seconds, frame_latency_ms_p50=float(np.percentile(fl, 50)),
+                             frame_latency_ms_p99=float(np.percentile(fl, 99)),
+                             request_latency_ms_p50=float(np.percentile(rl, 50)),
+                             request_latency_ms_p99=float(np.percentile(rl, 99)), max_concurrent=max_conc,
+                             steps=steps)
+        return (stats, outputs) if collect else stats
+
+    def close(self):
+        self.eng.destroy()
