@@ -181,12 +181,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-stream line")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(streams would shard across GPUs with no exchange)"}))
         sys.exit(2)
     from h261enc_cuda_amd import Engine
+
+    if args.copy_probe:
+        import torch
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
 
     specs = batched_streams()
     # warmup: a few steps of a small engine (module load, kernels, graph)
