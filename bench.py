@@ -148,6 +148,21 @@ def kernel_profile(Engine, specs, steps=12):
     return t
 
 
+def copy_probe(nbytes):
+    """one device-to-device copy of nbytes (the HIP runtime's copy kernel):
+    the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE"""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def cpu_baseline(frames=30, workers=None):
     """The oracle (C restatement of the reference path) on the host cores:
     one CIF stream of the plain configuration per worker process."""
@@ -191,11 +206,7 @@ def main():
     from h261enc_cuda_amd import Engine
 
     if args.copy_probe:
-        import torch
-        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
+        copy_probe(256 << 20)
 
     specs = batched_streams()
     # warmup: a few steps of a small engine (module load, kernels, graph)

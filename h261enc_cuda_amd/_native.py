@@ -50,7 +50,11 @@ EXPORTS = [
     "h261g_engine_sync", "h261g_stream_info", "h261g_stream_read_macroblocks", "h261g_stream_read_recon",
     "h261g_stream_read_me", "h261g_stream_read_bitstream", "h261g_stream_read_input",
     "h261g_engine_profile", "h261g_engine_kernel_times", "h261g_content_frame_host", "h261g_write_picture",
+    "h261g_engine_config",
 ]
+OPT_ME_VARIANT = 1
+OPT_MBS_PER_WAVE = 2
+OPT_FUSED_TAILS = 3
 
 
 class StreamParams(ctypes.Structure):
@@ -89,6 +93,7 @@ def load() -> ctypes.CDLL:
         "h261gpu_error_string": (ctypes.c_char_p, [ctypes.c_int]),
         "h261g_engine_create": (ctypes.c_int, [ctypes.POINTER(P), i32, i64, i32]),
         "h261g_engine_destroy": (ctypes.c_int, [P]),
+        "h261g_engine_config": (ctypes.c_int, [P, i32, i32]),
         "h261g_stream_footprint": (i64, [i32, i32]),
         "h261g_stream_open": (ctypes.c_int, [P, ctypes.POINTER(StreamParams)]),
         "h261g_stream_close": (ctypes.c_int, [P, i32]),

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/h261gpu.h"
+#include "frame_control.h"
 #include "h261gpu_internal.h"
 
 namespace h261g {
@@ -36,51 +37,13 @@ int generate_frames_device(const h261g_content_params_t &c, int width, int heigh
 
 namespace {
 
-enum : int32_t {
-    kStreamSeqInput = 4, /* input is a device frame sequence */
-};
-
-/* device-side per-stream input cursor */
-struct SeqState {
-    uint8_t *base;     /* frame 0 of the sequence */
-    int64_t frame_size;
-    int32_t remaining; /* frames left including the current one */
-    int32_t pp;        /* which reconstruction buffer is the reference */
-    uint8_t *recon[2];
-};
-
 __global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     StreamDesc &d = streams[s];
     if (!(d.flags & kStreamActive)) return;
-    SeqState &q = seq[s];
-    /* the reconstruction becomes the next reference (h261_main.c:247-249) */
-    q.pp ^= 1;
-    const size_t ysz = (size_t)d.width * d.height, csz = ysz / 4;
-    uint8_t *ref = q.recon[q.pp], *rec = q.recon[q.pp ^ 1];
-    d.ref[0] = ref;
-    d.ref[1] = ref + ysz;
-    d.ref[2] = ref + ysz + csz;
-    d.rec[0] = rec;
-    d.rec[1] = rec + ysz;
-    d.rec[2] = rec + ysz + csz;
-    d.frame_num++;
-    d.frame_type = H261_FRAME_TYPE_INTER;
-    if (d.flags & kStreamSeqInput) {
-        q.remaining--;
-        if (q.remaining <= 0) {
-            d.flags &= ~kStreamActive;
-        } else {
-            const uint8_t *cur = q.base + (size_t)q.frame_size * d.frame_num;
-            d.cur[0] = cur;
-            d.cur[1] = cur + ysz;
-            d.cur[2] = cur + ysz + csz;
-        }
-    } else {
-        d.flags &= ~kStreamActive; /* a host-fed stream waits for its next frame */
-    }
+    fc::advance_one(d, seq[s]);
 }
 
 bool ok(hipError_t e, const char *what)
@@ -113,6 +76,8 @@ struct h261g_engine {
     int max_streams = 0;
     int64_t budget = 0, used = 0;
     int mbs_per_wave = 1;
+    int me_variant = kMeTwoStage;
+    int fused_tails = 1;
     hipStream_t stream = nullptr;
     std::vector<Slot> slots;
     /* device arrays indexed by stream slot */
@@ -200,12 +165,14 @@ void invalidate_graph(h261g_engine *e)
 
 int stream_count(const h261g_engine *e) { return e->max_streams; }
 
-/* the six launches of one frame step */
+/* one frame step: two launches with the per-stream work fused into their
+ * tails (default), or the six standalone launches */
 int enqueue_step(h261g_engine *e, bool timed)
 {
     const int ns = stream_count(e);
     const int n_mb = e->mb_used;
     hipStream_t st = e->stream;
+    const int fused = e->fused_tails;
     if (timed) (void)hipEventRecord(e->ev[0], st);
     MeLaunch ml{};
     ml.streams = e->desc;
@@ -215,9 +182,11 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.mode = e->mode;
     ml.rc = e->rc;
     ml.skip_search_on_intra = 1;
+    ml.variant = e->me_variant;
+    ml.fuse_tail = fused;
     if (launch_me(ml, st)) return H261G_ERR_DEVICE;
     if (timed) (void)hipEventRecord(e->ev[1], st);
-    if (launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
+    if (!fused && launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
     if (timed) (void)hipEventRecord(e->ev[2], st);
     MbLaunch bl{};
     bl.streams = e->desc;
@@ -229,12 +198,16 @@ int enqueue_step(h261g_engine *e, bool timed)
     bl.side = e->side;
     bl.levels = e->levels;
     bl.mbs_per_wave = e->mbs_per_wave;
+    bl.fuse_tail = fused;
+    bl.seq = e->seq;
     if (launch_mb(bl, st)) return H261G_ERR_DEVICE;
     if (timed) (void)hipEventRecord(e->ev[3], st);
-    if (launch_denoise_update(e->dn, e->desc, ns, st)) return H261G_ERR_DEVICE;
-    if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st)) return H261G_ERR_DEVICE;
-    hipLaunchKernelGGL(advance_kernel, dim3((ns + 63) / 64), dim3(64), 0, st, e->desc, e->seq, ns);
-    if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
+    if (!fused) {
+        if (launch_denoise_update(e->dn, e->desc, ns, st)) return H261G_ERR_DEVICE;
+        if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st)) return H261G_ERR_DEVICE;
+        hipLaunchKernelGGL(advance_kernel, dim3((ns + 63) / 64), dim3(64), 0, st, e->desc, e->seq, ns);
+        if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
+    }
     return H261G_OK;
 }
 
@@ -261,6 +234,29 @@ Slot *get_slot(h261g_engine *e, int id)
 }  // namespace
 
 extern "C" {
+
+int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
+{
+    if (!e) return H261G_ERR_ARG;
+    switch (option) {
+    case H261G_OPT_ME_VARIANT:
+        if (value != kMeTwoStage && value != kMeSurface) return H261G_ERR_ARG;
+        e->me_variant = value;
+        break;
+    case H261G_OPT_MBS_PER_WAVE:
+        if (value < 1 || value > 64) return H261G_ERR_ARG;
+        e->mbs_per_wave = value;
+        break;
+    case H261G_OPT_FUSED_TAILS:
+        if (value != 0 && value != 1) return H261G_ERR_ARG;
+        e->fused_tails = value;
+        break;
+    default:
+        return H261G_ERR_ARG;
+    }
+    e->graph_valid = false;
+    return H261G_OK;
+}
 
 int64_t h261g_stream_footprint(int32_t width, int32_t height)
 {

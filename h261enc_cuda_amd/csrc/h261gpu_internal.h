@@ -45,6 +45,16 @@ struct StreamDesc {
 enum : int32_t {
     kStreamActive = 1,       /* encode a frame for this stream in this step */
     kStreamDeviceQuant = 2,  /* picture quantiser comes from RcState.pic_quant */
+    kStreamSeqInput = 4,     /* input is a device frame sequence */
+};
+
+/* device-side per-stream input cursor and reconstruction ping-pong */
+struct SeqState {
+    uint8_t *base;      /* frame 0 of a device frame sequence */
+    int64_t frame_size; /* bytes of one I420 frame */
+    int32_t remaining;  /* frames left including the current one */
+    int32_t pp;         /* which reconstruction buffer is the reference */
+    uint8_t *recon[2];
 };
 
 /* motion estimation result of one MB; same fields as me_gpu_mb_t */
@@ -92,7 +102,9 @@ struct RcState {
     int32_t cost_accum;  /* sum of MD costs of the current frame */
     int32_t frame_bytes; /* bytes of the last coded frame */
     int32_t frame_bits;
-    int32_t pad[3];
+    int32_t me_done;     /* MBs of this step through motion estimation (fused tail) */
+    int32_t mb_done;     /* MBs of this step through the macroblock coder (fused tail) */
+    int32_t pad;
 };
 
 /* ---- launchers (implemented in the .hip files) ---- */
@@ -109,22 +121,31 @@ struct MeLaunch {
     MbMode *mode;       /* optional [n_mb]: device mode decision */
     RcState *rc;        /* optional per-stream: cost accumulation */
     int32_t skip_search_on_intra; /* 1: intra frames only compute the intra cost */
+    int32_t variant;              /* kMeTwoStage or kMeSurface */
+    int32_t fuse_tail;            /* 1: the last MB of a stream runs rate-control begin */
+};
+enum : int32_t {
+    kMeTwoStage = 0, /* the reference's candidates only, one v_sad_u8 lane per candidate */
+    kMeSurface = 1,  /* full +-15 SAD surface by v_qsad_pk_u16_u8, then the same selection */
 };
 int launch_me(const MeLaunch &p, void *stream);
 
 /* one wave per MB: MC/loop filter, DCT, denoise, quant, dequant, IDCT,
  * reconstruction; statistics for the denoise update; bit counts. */
 struct MbLaunch {
-    const StreamDesc *streams;
+    StreamDesc *streams; /* written by the fused tail (advance) */
     const int32_t *mb_stream;
     int32_t n_mb;
     const MbMode *mode;           /* [n_mb] */
     DenoiseState *denoise;        /* per stream */
-    const RcState *rc;            /* per stream, used with kStreamDeviceQuant */
+    RcState *rc;                  /* per stream: kStreamDeviceQuant, fused tail */
     MbSide *side;                 /* optional [n_mb] */
     int16_t *levels;              /* optional [n_mb][6][64] */
     h261_macroblock_t *mb_struct; /* optional [n_mb] legacy layout */
     int32_t mbs_per_wave;         /* MBs coded by one wave in sequence */
+    int32_t fuse_tail;            /* 1: the last wave of a stream runs the denoise update,
+                                     frame size, rate-control update and advance */
+    SeqState *seq;                /* per stream, with fuse_tail */
 };
 int launch_mb(const MbLaunch &p, void *stream);
 
@@ -139,18 +160,6 @@ int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int n
  * size, rate-control update */
 int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, const MbSide *side,
                       int32_t *frame_bytes_out, void *stream);
-
-/* synthetic content (see content.hip) */
-struct ContentParams {
-    uint64_t seed;
-    int32_t width, height;
-    int32_t off_x, off_y; /* global texture offset of this frame */
-    int32_t frame;        /* frame index (noise) */
-    int32_t sigma;        /* noise standard deviation, integer */
-    const int16_t *mb_offset; /* optional per-MB (dx,dy) offsets added to off_x/off_y */
-};
-int launch_content(const ContentParams *params_dev, const int32_t *n_params, uint8_t *const *planes,
-                   int n_streams, int max_w, int max_h, void *stream);
 
 }  // namespace h261g
 

@@ -307,6 +307,9 @@ void h261_gpu_device_me(me_gpu_t *me)
     L.result = b->me_dev;
     L.mv_start = (int8_t *)me->ps_starting_vector_device;
     L.skip_search_on_intra = 0;
+    /* H261GPU_ME_VARIANT=0|1 selects the search kernel (same results) */
+    const char *v = getenv("H261GPU_ME_VARIANT");
+    L.variant = v && v[0] == '1' ? kMeSurface : kMeTwoStage;
     if (launch_me(L, b->stream) != 0) {
         check(hipGetLastError(), "me_kernel launch");
         return;

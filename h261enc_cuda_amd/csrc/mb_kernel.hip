@@ -25,6 +25,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include "frame_control.h"
 #include "h261_vlc.h"
 #include "h261gpu_internal.h"
 
@@ -128,7 +129,63 @@ __device__ __forceinline__ int tcoeff_bits(int run, int level)
     return kEscapeBits;
 }
 
-__global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
+/* the end of a stream of a frame step, run by the wave that codes its last
+ * MB: denoise update, exact frame size, rate-control update, advance */
+__device__ __forceinline__ void frame_tail(const MbLaunch &p, int s, int lane)
+{
+    StreamDesc &d = p.streams[s];
+    DenoiseState *dn = p.denoise + s;
+    const int ci = dn->inter_count, ca = dn->intra_count;
+    fc::denoise_update_lane(dn, lane, ci, ca);
+    __syncthreads();
+    RcState &rc = p.rc[s];
+    const int bits = fc::frame_bits_wave(d, p.side, rc.pic_quant, lane);
+    if (lane == 0) {
+        dn->inter_count = 0;
+        dn->intra_count = 0;
+        const int bytes = (bits + 7) >> 3;
+        rc.frame_bits = bits;
+        rc.frame_bytes = bytes;
+        fc::rc_update_one(rc, d.frame_type, bytes);
+        rc.mb_done = 0;
+        fc::advance_one(d, p.seq[s]);
+    }
+}
+
+/* flushes this wave's denoise statistics of stream s (LDS -> global
+ * atomics) and, with fused tails, counts its MBs; the wave that completes
+ * the stream's MB count runs the frame tail */
+__device__ __forceinline__ void flush_segment(const MbLaunch &p, int s, int count, int32_t (*stat)[64], int32_t *scount,
+                                              int lane)
+{
+    __syncthreads();
+    DenoiseState *dn = p.denoise + s;
+    if (stat[0][lane]) atomicAdd(&dn->intra_noise[lane], stat[0][lane]);
+    if (stat[1][lane]) atomicAdd(&dn->inter_noise[lane], stat[1][lane]);
+    if (lane == 0 && scount[0]) atomicAdd(&dn->intra_count, scount[0]);
+    if (lane == 1 && scount[1]) atomicAdd(&dn->inter_count, scount[1]);
+    __syncthreads();
+    stat[0][lane] = 0;
+    stat[1][lane] = 0;
+    if (lane < 2) scount[lane] = 0;
+    if (p.fuse_tail) {
+        /* every lane's stores (levels, side, reconstruction) and statistic
+         * atomics are visible device-wide before the count */
+        __threadfence();
+        __syncthreads();
+        int done = 0;
+        if (lane == 0) done = atomicAdd(&p.rc[s].mb_done, count);
+        done = __shfl(done, 0, 64);
+        const StreamDesc &d = p.streams[s];
+        if (done + count == d.mb_w * d.mb_h) {
+            __threadfence(); /* acquire: the other waves' results */
+            frame_tail(p, s, lane);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 {
     __shared__ float tf[6 * kBlkPitch];
     __shared__ int32_t lv[6 * 64];
@@ -144,7 +201,7 @@ __global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
     stat[0][lane] = 0;
     stat[1][lane] = 0;
     if (lane < 2) scount[lane] = 0;
-    int flush_stream = -1;
+    int seg_count = 0; /* MBs of the current stream coded by this wave since the last flush */
 
     for (int it = 0; it < p.mbs_per_wave; it++) {
         const int g = blockIdx.x * p.mbs_per_wave + it;
@@ -153,21 +210,7 @@ __global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
         if (s < 0) continue; /* unmapped range of a closed stream */
         const StreamDesc &d = p.streams[s];
         if (!(d.flags & kStreamActive)) continue;
-        if (flush_stream >= 0 && s != flush_stream) {
-            /* a workgroup's MBs span two streams: flush the first one's statistics */
-            __syncthreads();
-            DenoiseState *dn = p.denoise + flush_stream;
-            if (stat[0][lane]) atomicAdd(&dn->intra_noise[lane], stat[0][lane]);
-            if (stat[1][lane]) atomicAdd(&dn->inter_noise[lane], stat[1][lane]);
-            if (lane == 0 && scount[0]) atomicAdd(&dn->intra_count, scount[0]);
-            if (lane == 1 && scount[1]) atomicAdd(&dn->inter_count, scount[1]);
-            __syncthreads();
-            stat[0][lane] = 0;
-            stat[1][lane] = 0;
-            if (lane < 2) scount[lane] = 0;
-            __syncthreads();
-        }
-        flush_stream = s;
+        seg_count++;
         const int m = g - d.mb_base;
         const MbMode md = p.mode[g];
         const int W = d.width;
@@ -177,6 +220,7 @@ __global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
         const bool intra = md.type == H261_MB_TYPE_INTRA;
         const bool filter = (md.flags & H261_MB_FILTER) != 0;
         const int t = intra ? 0 : 1;
+        (void)t;
 
         /* ---- phase 1: lane (b, r) ---- */
         const int plane = b < 4 ? 0 : b - 3;
@@ -194,47 +238,79 @@ __global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
             mvx = md.mvx / 2;
             mvy = md.mvy / 2;
         }
+        /* issue every global load of the MB before using any: the current
+         * row, three reference rows (rc-1, rc, rc+1 clamped to the block --
+         * the loop filter is 1-D at the block edges; an intra MB has a zero
+         * vector, so its addresses are in the frame as well) as aligned
+         * dwords, and this lane's eight denoise offsets */
+        const int t_off = intra ? 0 : 1;
+        uint32_t offw[8];
         int pred[8], res[8];
+        uint2 cw = make_uint2(0, 0);
+        uint32_t rw[3][3];
+        int o = 0;
         if (act) {
-            const uint8_t *cp = d.cur[plane] + (size_t)(py + rc) * stride + px;
-            uint2 cw = *(const uint2 *)cp;
+            const int32_t *offs = t_off == 0 ? p.denoise[s].intra_offset : p.denoise[s].inter_offset;
+#pragma unroll
+            for (int r = 0; r < 8; r++) offw[r] = (uint32_t)offs[rc + 8 * r];
+            cw = *(const uint2 *)(d.cur[plane] + (size_t)(py + rc) * stride + px);
+            const int rx = px + mvx;
+            const int a = rx & ~3;
+            o = rx & 3;
+            /* the third dword is needed only when o > 0, and then it lies in the row */
+            const int a2 = a + 8 <= stride - 4 ? a + 8 : a;
+            const uint8_t *rbase = d.ref[plane] + (size_t)(py + mvy) * stride;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int rr = clampi(rc + k - 1, 0, 7);
+                const uint8_t *q = rbase + (size_t)rr * stride;
+                rw[k][0] = *(const uint32_t *)(q + a);
+                rw[k][1] = *(const uint32_t *)(q + a + 4);
+                rw[k][2] = *(const uint32_t *)(q + a2);
+            }
+        }
+        if (act) {
             int cpel[8];
 #pragma unroll
             for (int x = 0; x < 4; x++) {
                 cpel[x] = (cw.x >> (8 * x)) & 0xff;
                 cpel[x + 4] = (cw.y >> (8 * x)) & 0xff;
             }
+            /* the 8 reference pels of rows rc-1, rc, rc+1 */
+            int a8[3][8];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const uint32_t lo = __builtin_amdgcn_alignbyte(rw[k][1], rw[k][0], o);
+                const uint32_t hi = __builtin_amdgcn_alignbyte(rw[k][2], rw[k][1], o);
+#pragma unroll
+                for (int x = 0; x < 4; x++) {
+                    a8[k][x] = (lo >> (8 * x)) & 0xff;
+                    a8[k][x + 4] = (hi >> (8 * x)) & 0xff;
+                }
+            }
             if (intra) {
 #pragma unroll
                 for (int x = 0; x < 8; x++) pred[x] = 0;
+            } else if (!filter) {
+#pragma unroll
+                for (int x = 0; x < 8; x++) pred[x] = a8[1][x];
             } else {
-                const uint8_t *rp = d.ref[plane] + (size_t)(py + mvy) * stride + px + mvx;
-                if (!filter) {
+                /* H.261 loop filter, separable [1 2 1]: 1-D at the block
+                 * edges, corners unfiltered (mbgpu:79-150) */
+                int h[3][8];
 #pragma unroll
-                    for (int x = 0; x < 8; x++) pred[x] = rp[rc * stride + x];
+                for (int k = 0; k < 3; k++) {
+                    h[k][0] = 4 * a8[k][0];
+#pragma unroll
+                    for (int x = 1; x < 7; x++) h[k][x] = a8[k][x - 1] + 2 * a8[k][x] + a8[k][x + 1];
+                    h[k][7] = 4 * a8[k][7];
+                }
+                if (rc == 0 || rc == 7) {
+#pragma unroll
+                    for (int x = 0; x < 8; x++) pred[x] = (h[1][x] + 2) >> 2;
                 } else {
-                    /* H.261 loop filter, separable [1 2 1]: 1-D at the
-                     * block edges, corners unfiltered (mbgpu:79-150) */
-                    int h[3][8];
 #pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        int rr = clampi(rc + k - 1, 0, 7);
-                        const uint8_t *q = rp + rr * stride;
-                        int a[8];
-#pragma unroll
-                        for (int x = 0; x < 8; x++) a[x] = q[x];
-                        h[k][0] = 4 * a[0];
-#pragma unroll
-                        for (int x = 1; x < 7; x++) h[k][x] = a[x - 1] + 2 * a[x] + a[x + 1];
-                        h[k][7] = 4 * a[7];
-                    }
-                    if (rc == 0 || rc == 7) {
-#pragma unroll
-                        for (int x = 0; x < 8; x++) pred[x] = (h[1][x] + 2) >> 2;
-                    } else {
-#pragma unroll
-                        for (int x = 0; x < 8; x++) pred[x] = (h[0][x] + 2 * h[1][x] + h[2][x] + 8) >> 4;
-                    }
+                    for (int x = 0; x < 8; x++) pred[x] = (h[0][x] + 2 * h[1][x] + h[2][x] + 8) >> 4;
                 }
             }
 #pragma unroll
@@ -270,12 +346,10 @@ __global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
 
             /* denoise with the previous frame's offsets, staged as UInt16
              * (mbgpu:525-528, 586-604) */
-            const DenoiseState &dn = p.denoise[s];
-            const int32_t *offs = intra ? dn.intra_offset : dn.inter_offset;
 #pragma unroll
             for (int r = 0; r < 8; r++) {
                 int i = c + 8 * r;
-                int off = (int)(uint16_t)offs[i];
+                int off = (int)(uint16_t)offw[r];
                 int cc = coef[r];
                 int a = cc < 0 ? -cc : cc;
                 atomicAdd(&stat[t][i], a);
@@ -442,36 +516,16 @@ __global__ __launch_bounds__(64) void mb_kernel(MbLaunch p)
         }
         if (lane == 0) atomicAdd(&scount[t], 6);
         __syncthreads();
+        /* the segment of stream s in this wave ends here when the wave's next
+         * MB belongs to another stream (or there is none) */
+        const int gn = g + 1;
+        const bool last = it + 1 == p.mbs_per_wave || gn >= p.n_mb || p.mb_stream[gn] != s;
+        if (last) {
+            flush_segment(p, s, seg_count, stat, scount, lane);
+            seg_count = 0;
+        }
     }
     (void)sblk;
-    __syncthreads();
-    if (flush_stream >= 0) {
-        DenoiseState *dn = p.denoise + flush_stream;
-        if (stat[0][lane]) atomicAdd(&dn->intra_noise[lane], stat[0][lane]);
-        if (stat[1][lane]) atomicAdd(&dn->inter_noise[lane], stat[1][lane]);
-        if (lane == 0 && scount[0]) atomicAdd(&dn->intra_count, scount[0]);
-        if (lane == 1 && scount[1]) atomicAdd(&dn->inter_count, scount[1]);
-    }
-}
-
-/* macroblocks_update_dct_denoise (src/mbgpu_kernels.cuh:478-503) */
-__global__ __launch_bounds__(64) void denoise_update_kernel(DenoiseState *dn_all, const StreamDesc *streams)
-{
-    const int s = blockIdx.x;
-    if (!(streams[s].flags & kStreamActive)) return;
-    DenoiseState *dn = dn_all + s;
-    const int i = threadIdx.x;
-    const int ni = dn->inter_noise[i], na = dn->intra_noise[i];
-    const int ci = dn->inter_count, ca = dn->intra_count;
-    dn->inter_offset[i] = (ci * dn->denoise + ni / 2) / (ni + 1);
-    dn->intra_offset[i] = (ca * dn->denoise + na / 2) / (na + 1);
-    dn->inter_noise[i] = 0;
-    dn->intra_noise[i] = 0;
-    __syncthreads();
-    if (i == 0) {
-        dn->inter_count = 0;
-        dn->intra_count = 0;
-    }
 }
 
 }  // namespace
@@ -484,13 +538,6 @@ int launch_mb(const MbLaunch &p, void *stream)
     q.mbs_per_wave = per;
     const int grid = (p.n_mb + per - 1) / per;
     hipLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, q);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_denoise_update(DenoiseState *dn, const StreamDesc *streams, int n_streams, void *stream)
-{
-    if (n_streams <= 0) return 0;
-    hipLaunchKernelGGL(denoise_update_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, dn, streams);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -22,6 +22,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include "frame_control.h"
 #include "h261gpu_internal.h"
 
 namespace h261g {
@@ -120,10 +121,43 @@ __device__ __forceinline__ int satd4x4(const int d[16])
     return s;
 }
 
+/* the full SAD surface of one MB for dy, dx in [-15, 15] (dx = -16 is a
+ * padding column), surf[(dy + 15) * 32 + dx + 16].  Lane (dy, half) covers
+ * 16 horizontal offsets: window dwords g0..g0+7 of each row (g0 = 4 half,
+ * 16-byte aligned loads) against the 4 dwords of the current row, one
+ * v_qsad_pk_u16_u8 per 4-byte column and group of 4 offsets. */
+__device__ __forceinline__ void sad_surface(const uint32_t *win, const uint32_t *curw, uint16_t *surf, int lane)
+{
+    if (lane >= 62) return;
+    const int dyi = lane >> 1, half = lane & 1;
+    const uint32_t *row = win + dyi * kWinDw + 4 * half;
+    uint64_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+#pragma unroll 2
+    for (int k = 0; k < 16; k++) {
+        const uint4 a = *(const uint4 *)(row + k * kWinDw);
+        const uint4 b = *(const uint4 *)(row + k * kWinDw + 4);
+        const uint4 c = *(const uint4 *)(curw + 4 * k);
+        const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        const uint32_t cq[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            acc0 = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[q + 1] << 32) | w[q], cq[q], acc0);
+            acc1 = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[q + 2] << 32) | w[q + 1], cq[q], acc1);
+            acc2 = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[q + 3] << 32) | w[q + 2], cq[q], acc2);
+            acc3 = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)w[q + 4] << 32) | w[q + 3], cq[q], acc3);
+        }
+    }
+    uint4 *out = (uint4 *)(surf + dyi * 32 + 16 * half);
+    out[0] = make_uint4((uint32_t)acc0, (uint32_t)(acc0 >> 32), (uint32_t)acc1, (uint32_t)(acc1 >> 32));
+    out[1] = make_uint4((uint32_t)acc2, (uint32_t)(acc2 >> 32), (uint32_t)acc3, (uint32_t)(acc3 >> 32));
+}
+
+template <int kVariant>
 __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
 {
-    __shared__ uint32_t win[kWinRows * kWinDw];
-    __shared__ uint32_t curw[64];
+    __shared__ __align__(16) uint32_t win[kWinRows * kWinDw];
+    __shared__ __align__(16) uint32_t curw[64];
+    __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
 
     const int g = blockIdx.x;
     const int lane = threadIdx.x;
@@ -141,27 +175,46 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
     const int lx0 = max(bx - 15, 0) - bx, lx1 = min(bx + 15, (d.mb_w - 1) * 16) - bx;
     const int ly0 = max(by - 15, 0) - by, ly1 = min(by + 15, (d.mb_h - 1) * 16) - by;
 
-    /* stage the window; pels outside the frame are never addressed */
-    for (int i = lane; i < kWinRows * kWinDw; i += 64) {
-        int r = i / kWinDw, c4 = i - r * kWinDw;
-        int fy = by - 15 + r, fx = bx - 16 + 4 * c4;
-        uint32_t v = 0;
-        if (fy >= 0 && fy < H && fx >= 0 && fx < W) v = *(const uint32_t *)(ref + (size_t)fy * W + fx);
-        win[i] = v;
+    /* stage the window: all loads issued before any wait.  Window dwords
+     * outside the frame are loaded from a clamped in-frame address; no
+     * in-limit candidate or filter tap ever reads them. */
+    constexpr int kWinWords = kWinRows * kWinDw; /* 552 */
+    constexpr int kPerLane = (kWinWords + 63) / 64;
+    uint32_t staged[kPerLane];
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const int i = lane + 64 * j;
+        const int r = i / kWinDw, c4 = i - r * kWinDw;
+        const int fy = min(max(by - 15 + r, 0), H - 1);
+        const int fx = min(max(bx - 16 + 4 * c4, 0), W - 4);
+        staged[j] = *(const uint32_t *)(ref + (size_t)fy * W + fx);
     }
-    curw[lane] = *(const uint32_t *)(cur + (size_t)(by + (lane >> 2)) * W + bx + 4 * (lane & 3));
+    const uint32_t cw = *(const uint32_t *)(cur + (size_t)(by + (lane >> 2)) * W + bx + 4 * (lane & 3));
+#pragma unroll
+    for (int j = 0; j < kPerLane; j++) {
+        const int i = lane + 64 * j;
+        if (i < kWinWords) win[i] = staged[j];
+    }
+    curw[lane] = cw;
     __syncthreads();
 
     const bool search = !(p.skip_search_on_intra && d.frame_type == H261_FRAME_TYPE_INTRA);
     int mvx = 0, mvy = 0;
     if (search) {
+        if (kVariant == kMeSurface) {
+            sad_surface(win, curw, surf, lane);
+            __syncthreads();
+        }
+        auto cost_of = [&](int vx, int vy) -> uint32_t {
+            if (!(vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1)) return 65535u;
+            return kVariant == kMeSurface ? (uint32_t)surf[(vy + 15) * 32 + vx + 16] : sad16(win, curw, vx, vy);
+        };
         /* stage 1: coarse grid */
         uint32_t best = 0xffffffffu;
         for (int ci = lane; ci < 121; ci += 64) {
             int tx = ci % 11, ty = ci / 11;
             int vx = coarse_vec(tx), vy = coarse_vec(ty);
-            uint32_t cost = 65535u;
-            if (vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1) cost = sad16(win, curw, vx, vy);
+            uint32_t cost = cost_of(vx, vy);
             uint32_t key = (cost << 8) | morton((uint32_t)tx, (uint32_t)ty);
             best = key < best ? key : best;
         }
@@ -175,8 +228,7 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
         if (lane < 49) {
             int tx = lane % 7, ty = lane / 7;
             int vx = sx + tx - 3, vy = sy + ty - 3;
-            uint32_t cost = 65535u;
-            if (vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1) cost = sad16(win, curw, vx, vy);
+            uint32_t cost = cost_of(vx, vy);
             best = (cost << 8) | morton((uint32_t)tx, (uint32_t)ty);
         }
         best = wave_min_u32(best);
@@ -260,7 +312,20 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
                 if (mvx != 0 || mvy != 0) md.flags |= H261_MB_MC;
             }
             p.mode[g] = md;
-            if (p.rc) atomicAdd(&p.rc[s].cost_accum, md.cost);
+            if (p.rc) {
+                atomicAdd(&p.rc[s].cost_accum, md.cost);
+                if (p.fuse_tail) {
+                    /* the last MB of the stream through ME starts its picture:
+                     * picture cost, rate control, quantiser */
+                    __threadfence();
+                    const int done = atomicAdd(&p.rc[s].me_done, 1);
+                    if (done == d.mb_w * d.mb_h - 1) {
+                        __threadfence();
+                        fc::rc_begin_one(p.rc[s], d);
+                        p.rc[s].me_done = 0;
+                    }
+                }
+            }
         }
     }
 }
@@ -270,7 +335,10 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
 int launch_me(const MeLaunch &p, void *stream)
 {
     if (p.n_mb <= 0) return 0;
-    hipLaunchKernelGGL(me_kernel, dim3(p.n_mb), dim3(64), 0, (hipStream_t)stream, p);
+    if (p.variant == kMeSurface)
+        hipLaunchKernelGGL(me_kernel<kMeSurface>, dim3(p.n_mb), dim3(64), 0, (hipStream_t)stream, p);
+    else
+        hipLaunchKernelGGL(me_kernel<kMeTwoStage>, dim3(p.n_mb), dim3(64), 0, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -46,12 +46,18 @@ def write_picture(mbs: np.ndarray, width: int, height: int, frame_num: int, quan
 class Engine:
     """Many independent QCIF/CIF streams, one frame step at a time."""
 
-    def __init__(self, max_streams: int = 16, hbm_budget: int = 0, mbs_per_wave: int = 0):
+    def __init__(self, max_streams: int = 16, hbm_budget: int = 0, mbs_per_wave: int = 0,
+                 me_variant: Optional[int] = None):
         self.lib = nat.load()
         h = ctypes.c_void_p()
         check(self.lib.h261g_engine_create(ctypes.byref(h), max_streams, hbm_budget, mbs_per_wave), "engine_create")
         self.h = h
         self.sizes = {}
+        if me_variant is not None:
+            self.config(nat.OPT_ME_VARIANT, me_variant)
+
+    def config(self, option: int, value: int) -> None:
+        check(self.lib.h261g_engine_config(self.h, option, value), "engine_config")
 
     def open(self, width: int, height: int, denoise: int = 5, init_quant: int = 10, bitrate: int = 512000,
              vbv_size: int = 512000, fixed_quant: int = 0) -> int:

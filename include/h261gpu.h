@@ -25,6 +25,13 @@
 
 #include "h261gpu_abi.h"
 
+/* exported from libh261gpu.so (built with -fvisibility=hidden) */
+#if defined(__GNUC__)
+#define H261G_API __attribute__((visibility("default")))
+#else
+#define H261G_API
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -34,39 +41,39 @@ extern "C" {
 /* ------------------------------------------------------------------ */
 
 /* device selection; src/gpu_device.cu:58-87 */
-Void h261_gpu_init(void);
+H261G_API Void h261_gpu_init(void);
 
 /* current / reference planes; src/gpu_device.cu:90-215.  The reference
  * holds them in six cudaArray textures; here they are linear device planes
  * with stride = width.  set_* copy synchronously from the caller's buffer. */
-Void h261_gpu_device_init_textures(Int32 i_width, Int32 i_height);
-Void h261_gpu_device_deinit_textures(void);
-Void h261_gpu_device_set_current(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
-Void h261_gpu_device_set_current_chroma_cb(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
-Void h261_gpu_device_set_current_chroma_cr(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
-Void h261_gpu_device_set_reference(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
-Void h261_gpu_device_set_reference_chroma_cb(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
-Void h261_gpu_device_set_reference_chroma_cr(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
+H261G_API Void h261_gpu_device_init_textures(Int32 i_width, Int32 i_height);
+H261G_API Void h261_gpu_device_deinit_textures(void);
+H261G_API Void h261_gpu_device_set_current(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
+H261G_API Void h261_gpu_device_set_current_chroma_cb(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
+H261G_API Void h261_gpu_device_set_current_chroma_cr(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
+H261G_API Void h261_gpu_device_set_reference(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
+H261G_API Void h261_gpu_device_set_reference_chroma_cb(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
+H261G_API Void h261_gpu_device_set_reference_chroma_cr(UInt8 *pui8_frame_data, Int32 i_width, Int32 i_height);
 
 /* motion estimation; src/gpu_device.cu:286-432.  h261_gpu_device_me fills
  * ps_me_gpu->prgs_macroblocks_result[] (costs + vector per MB). */
-void h261_gpu_device_init_me(me_gpu_t **pps_me_gpu, Int32 i_width, Int32 i_height);
-void h261_gpu_device_deinit_me(me_gpu_t **pps_me_gpu);
-void h261_gpu_device_me(me_gpu_t *ps_me_gpu);
+H261G_API void h261_gpu_device_init_me(me_gpu_t **pps_me_gpu, Int32 i_width, Int32 i_height);
+H261G_API void h261_gpu_device_deinit_me(me_gpu_t **pps_me_gpu);
+H261G_API void h261_gpu_device_me(me_gpu_t *ps_me_gpu);
 
 /* macroblock coder; src/gpu_device.cu:442-815.  encode fills
  * ps_ctx->ps_macroblocks->ps_macroblocks[] (type, flags, quant, mv, cbp,
  * levels); decode fills pui8_reconstructed_{Y,Cb,Cr}_host. */
-Void h261_init_macroblocks(h261_macroblocks_t **pps_mbs, Int32 i_width, Int32 i_height, Int32 i_denoise);
-Void h261_deinit_macroblocks(h261_macroblocks_t *ps_mbs);
-Void h261_gpu_device_encode_macroblocks(h261_context_t *ps_ctx);
-Void h261_gpu_device_decode_macroblocks(h261_context_t *ps_ctx);
+H261G_API Void h261_init_macroblocks(h261_macroblocks_t **pps_mbs, Int32 i_width, Int32 i_height, Int32 i_denoise);
+H261G_API Void h261_deinit_macroblocks(h261_macroblocks_t *ps_mbs);
+H261G_API Void h261_gpu_device_encode_macroblocks(h261_context_t *ps_ctx);
+H261G_API Void h261_gpu_device_decode_macroblocks(h261_context_t *ps_ctx);
 
 /* last error of any call of this library (0 = none); clears it.  The
  * reference only prints cudaGetLastError (gpu_device.cu:430, 727, 813). */
-int h261gpu_last_error(void);
+H261G_API int h261gpu_last_error(void);
 /* human-readable text of an error code */
-const char *h261gpu_error_string(int err);
+H261G_API const char *h261gpu_error_string(int err);
 
 /* ------------------------------------------------------------------ */
 /* Part 2: multi-stream engine                                         */
@@ -104,35 +111,44 @@ typedef struct {
 /* max_streams: stream slots; hbm_budget: bytes of device memory the engine
  * may use for stream state (0 = no limit).  mbs_per_wave: MBs one wave of
  * the macroblock coder codes in sequence (0 = chosen by the autotuner). */
-int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_budget, int32_t mbs_per_wave);
-int h261g_engine_destroy(h261g_engine_t *e);
+H261G_API int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_budget, int32_t mbs_per_wave);
+H261G_API int h261g_engine_destroy(h261g_engine_t *e);
+
+/* engine options (h261g_engine_config) */
+#define H261G_OPT_ME_VARIANT 1   /* 0: the reference's candidates, v_sad_u8 per candidate;
+                                    1: full +-15 SAD surface by v_qsad_pk_u16_u8 (same selection,
+                                    same results) */
+#define H261G_OPT_MBS_PER_WAVE 2 /* macroblocks one wave of the macroblock coder codes */
+#define H261G_OPT_FUSED_TAILS 3  /* 1 (default): per-stream frame control runs in the tails of the
+                                    two frame-step kernels; 0: as four separate launches */
+H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* device bytes one stream of the given size occupies */
-int64_t h261g_stream_footprint(int32_t width, int32_t height);
+H261G_API int64_t h261g_stream_footprint(int32_t width, int32_t height);
 
 /* opens a stream; returns its id (>= 0) or an H261G_ERR_* code */
-int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p);
-int h261g_stream_close(h261g_engine_t *e, int32_t id);
+H261G_API int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p);
+H261G_API int h261g_stream_close(h261g_engine_t *e, int32_t id);
 
 /* input of the next step: copy host planes (Y, Cb, Cr with stride = width
  * resp. width/2) */
-int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_t *y, const uint8_t *cb, const uint8_t *cr);
+H261G_API int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_t *y, const uint8_t *cb, const uint8_t *cr);
 /* device-resident input: generates frames 0..n_frames-1 of synthetic
  * content into HBM; the stream then codes one per step until the sequence
  * ends (the stream stays open, inactive, afterwards) */
-int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_content_params_t *c, int32_t n_frames);
+H261G_API int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_content_params_t *c, int32_t n_frames);
 /* number of frames a stream has coded (synchronises) */
-int h261g_stream_frames_done(h261g_engine_t *e, int32_t id);
+H261G_API int h261g_stream_frames_done(h261g_engine_t *e, int32_t id);
 /* re-packs the macroblock index space after streams closed (synchronises) */
-int h261g_engine_compact(h261g_engine_t *e);
+H261G_API int h261g_engine_compact(h261g_engine_t *e);
 
 /* encodes one frame for every stream that has an input; asynchronous.
  * Returns the number of streams coded. */
-int h261g_engine_step(h261g_engine_t *e);
+H261G_API int h261g_engine_step(h261g_engine_t *e);
 /* steps repeatedly: n_steps frame steps replayed from a captured HIP graph
  * for streams with device-generated content */
-int h261g_engine_run(h261g_engine_t *e, int32_t n_steps);
-int h261g_engine_sync(h261g_engine_t *e);
+H261G_API int h261g_engine_run(h261g_engine_t *e, int32_t n_steps);
+H261G_API int h261g_engine_sync(h261g_engine_t *e);
 
 /* results of the last step of a stream (synchronise first) */
 typedef struct {
@@ -141,36 +157,36 @@ typedef struct {
     int32_t quant;      /* picture quantiser */
     int32_t bytes;      /* exact H.261 size of the frame in bytes */
 } h261g_frame_info_t;
-int h261g_stream_info(h261g_engine_t *e, int32_t id, h261g_frame_info_t *info);
+H261G_API int h261g_stream_info(h261g_engine_t *e, int32_t id, h261g_frame_info_t *info);
 /* macroblocks of the last frame in the reference's h261_macroblock_t
  * layout (n_mb entries) */
-int h261g_stream_read_macroblocks(h261g_engine_t *e, int32_t id, h261_macroblock_t *mbs);
+H261G_API int h261g_stream_read_macroblocks(h261g_engine_t *e, int32_t id, h261_macroblock_t *mbs);
 /* reconstruction of the last frame */
-int h261g_stream_read_recon(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
+H261G_API int h261g_stream_read_recon(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
 /* motion estimation results of the last frame (n_mb entries) */
-int h261g_stream_read_me(h261g_engine_t *e, int32_t id, me_gpu_mb_t *me);
+H261G_API int h261g_stream_read_me(h261g_engine_t *e, int32_t id, me_gpu_mb_t *me);
 /* the last frame's H.261 bitstream written on the host from the device's
  * levels and side information; returns its byte count (equal to
  * h261g_frame_info_t.bytes) or an error */
-int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int32_t cap);
+H261G_API int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int32_t cap);
 /* the current input planes (for checking device-generated content) */
-int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
+H261G_API int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
 
 /* per-kernel device time accumulated since the last reset, in ms
  * (HIP events around each launch; enabled by h261g_engine_profile(e, 1)) */
-int h261g_engine_profile(h261g_engine_t *e, int32_t enable);
-int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, double *other_ms,
+H261G_API int h261g_engine_profile(h261g_engine_t *e, int32_t enable);
+H261G_API int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, double *other_ms,
                               int64_t *launches);
 
 /* synthetic content on the host (the same pels the device generator makes)
  * for frame `frame` of a stream */
-int h261g_content_frame_host(const h261g_content_params_t *c, int32_t width, int32_t height, int32_t frame,
+H261G_API int h261g_content_frame_host(const h261g_content_params_t *c, int32_t width, int32_t height, int32_t frame,
                              uint8_t *y, uint8_t *cb, uint8_t *cr);
 
 /* the H.261 writer over a frame's macroblocks, on the host: picture header,
  * GOBs, MBs, TCOEFF, in the syntax of src/h261_picture.c, src/h261_gob.c,
  * src/h261_macroblock.c.  Returns the byte count or an error. */
-int h261g_write_picture(const h261_macroblock_t *mbs, int32_t width, int32_t height, int32_t frame_num,
+H261G_API int h261g_write_picture(const h261_macroblock_t *mbs, int32_t width, int32_t height, int32_t frame_num,
                         int32_t quant, uint8_t *out, int32_t cap);
 
 #ifdef __cplusplus

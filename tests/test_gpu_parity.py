@@ -42,8 +42,10 @@ def assert_me_equal(got, want, what):
         assert bad.size == 0, f"{what}: {f} differs at MBs {bad[:10]}: got {got[f][bad[:5]]} want {want[f][bad[:5]]}"
 
 
+@pytest.mark.parametrize("variant", ["0", "1"], ids=["two-stage", "surface"])
 @pytest.mark.parametrize("size", [CIF, QCIF], ids=["cif", "qcif"])
-def test_legacy_me_matches_oracle(oracle, size):
+def test_legacy_me_matches_oracle(oracle, size, variant, monkeypatch):
+    monkeypatch.setenv("H261GPU_ME_VARIANT", variant)
     rng = np.random.default_rng(7)
     frames = seq(11, size, 4, sigma=2, motion=7)
     cases = [(frames[0], zeros(size), "frame0 vs zero ref")]
@@ -130,12 +132,13 @@ def run_engine_host_fed(eng, sid, frames):
     return out
 
 
+@pytest.mark.parametrize("variant", [0, 1], ids=["two-stage", "surface"])
 @pytest.mark.parametrize("size,fixed", [(CIF, 8), (QCIF, 0), (CIF, 0)], ids=["cif-q8", "qcif-rc", "cif-rc"])
-def test_engine_single_stream_matches_oracle(oracle, size, fixed):
+def test_engine_single_stream_matches_oracle(oracle, size, fixed, variant):
     w, h = size
     n = 10
     frames = seq(41, size, n, sigma=2, motion=7)
-    eng = Engine(max_streams=2)
+    eng = Engine(max_streams=2, me_variant=variant)
     sid = eng.open(w, h, fixed_quant=fixed)
     announce(f"engine single {size} fixed={fixed}")
     res = run_engine_host_fed(eng, sid, frames)
@@ -158,7 +161,7 @@ def test_engine_batched_matches_oracle_and_alone(oracle):
     specs = [(CIF, 101, 0, 256000), (QCIF, 102, 0, 64000), (CIF, 103, 8, 0), (QCIF, 104, 0, 384000),
              (QCIF, 105, 12, 0), (CIF, 106, 0, 1024000)]
     n = 8
-    eng = Engine(max_streams=8)
+    eng = Engine(max_streams=8, me_variant=1)
     sids = []
     for size, seed, fixed, rate in specs:
         sid = eng.open(*size, fixed_quant=fixed, bitrate=rate or 512000, vbv_size=rate or 512000)
@@ -226,3 +229,37 @@ def test_engine_graph_run_matches_stepwise():
         assert a.frames_done(sa) == b.frames_done(sb) == n
     a.destroy()
     b.destroy()
+
+
+def test_engine_launch_shapes_agree(oracle):
+    """Fused and unfused frame tails, 1/3/8 MBs per wave (workgroups that
+    span several streams) and both search kernels code identical bytes."""
+    from h261enc_cuda_amd._native import OPT_FUSED_TAILS, OPT_MBS_PER_WAVE
+    specs = [(QCIF, 301), (CIF, 302), (QCIF, 303), (QCIF, 304), (CIF, 305)]
+    n = 5
+    shapes = [(1, 1, 0), (0, 3, 0), (1, 8, 1), (0, 1, 1)]
+    results = []
+    for fused, mpw, variant in shapes:
+        announce(f"launch shape fused={fused} mbs_per_wave={mpw} variant={variant}")
+        eng = Engine(max_streams=len(specs), me_variant=variant)
+        eng.config(OPT_FUSED_TAILS, fused)
+        eng.config(OPT_MBS_PER_WAVE, mpw)
+        ids = []
+        for size, seed in specs:
+            sid = eng.open(*size, bitrate=128000, vbv_size=128000)
+            eng.generate_input(sid, seed, n, sigma=3, motion=15)
+            ids.append(sid)
+        out = []
+        for _ in range(n):
+            eng.step()
+            eng.sync()
+            out.append([eng.bitstream(sid) for sid in ids])
+        eng.destroy()
+        results.append(out)
+    for r in results[1:]:
+        assert r == results[0]
+    w, h = specs[1][0]
+    s = oracle.stream(w, h, bitrate=128000, vbv=128000)
+    for t in range(n):
+        f = content_frame(specs[1][1], w, h, t, sigma=3, motion=15)
+        assert s.encode(*f)[0] == results[0][t][1], f"frame {t}"
