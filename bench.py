@@ -4,15 +4,19 @@ Workload (the batched configuration): 256 CIF + 256 QCIF streams, each of a
 seed-drawn length uniform in 60..600 frames with a per-stream global motion
 in [-15, 15] px/frame, rate control on with a per-stream target uniform in
 64..384 kbit/s (VBV = target, initial quantiser 10).  Every frame of every
-stream is generated into HBM before the timed region; the timed region codes
-the whole job: K frame steps, each one pass of the engine over every active
-stream (motion estimation, mode decision, rate control, the fused
-macroblock coder, denoise update, exact frame size).
+stream is generated into HBM before the timed region.  After W untimed
+frame steps the timed region codes the rest of the job: K frame steps
+(default 600 - W), each one pass of the engine over every active stream
+(motion estimation, mode decision, rate control, the fused macroblock coder,
+denoise update, exact frame size).  A second, untimed pass repeats the same
+W + K steps with HIP events around each kernel launch: the per-kernel
+average durations behind the roofline.
 
 Also measured: the plain single stream (one CIF stream, 300 frames, fixed
-QUANT=8) frames/s and per-frame latency p50/p99, the per-kernel device
-times behind the roofline, and the oracle (the reference path restated in
-C) on the host cores as the CPU baseline.
+QUANT=8) frames/s and per-frame latency p50/p99; the hard-motion batch
+(1024 streams, per-MB local motion, scene cuts); the serving loop (Poisson
+arrivals, continuous batching, per-frame latency); and the oracle (the
+reference path restated in C) on the host cores as the CPU baseline.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -38,6 +42,24 @@ SAD_PEAK = 157.3e12  # byte-SADs/s: 256 CU x 64 lanes x 2.4 GHz x 4 (v_sad_u8)
 # side information for the macroblock coder)
 ME_BYTES = {396: 352 * 288 * 2 + 16 * 396, 99: 176 * 144 * 2 + 16 * 99}
 MB_BYTES = {396: 3 * 152064 + 396 * (768 + 16), 99: 3 * 38016 + 99 * (768 + 16)}
+
+
+# HBM bytes per macroblock actually moved by each kernel (FETCH_SIZE +
+# WRITE_SIZE per dispatch from two rocprofv3 --pmc passes over this
+# benchmark at --steps 8, each scaled by the same counter's reading for a
+# device copy of known size, divided by the dispatch's macroblock count).
+# Empty until measured: "traffic" is then null.
+TRAFFIC_PER_MB = {}
+
+
+def traffic_per_launch(kernel, algorithmic_bytes):
+    """HBM traffic of an average launch: measured bytes per MB times the
+    launch's MBs (the algorithmic bytes fix the MB count)"""
+    t = TRAFFIC_PER_MB.get(kernel)
+    if t is None:
+        return None
+    per_mb_alg = (ME_BYTES[396] if kernel == "me_kernel" else MB_BYTES[396]) / 396.0
+    return int(t * algorithmic_bytes / per_mb_alg)
 
 
 def coarse_valid_candidates(mb_w, mb_h):
@@ -70,32 +92,51 @@ def batched_streams(seed=2026):
     return specs
 
 
-def open_streams(eng, specs, sigma=2):
+def hard_streams(seed=2027):
+    """1024 mixed QCIF/CIF streams, per-macroblock local motion (random
+    affine warp + 10% random vectors), scene cuts every 50 frames, sigma 4"""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(1024):
+        cif = i < 512
+        specs.append(dict(size=(352, 288) if cif else (176, 144), length=int(rng.integers(60, 601)),
+                          bitrate=int(rng.integers(64, 385)) * 1000, seed=int(rng.integers(1, 2 ** 62)),
+                          motion=0, local=True, scene_cut=50, sigma=4))
+    return specs
+
+
+def open_streams(eng, specs, sigma=2, frames=None):
     ids = []
     for s in specs:
         sid = eng.open(*s["size"], bitrate=s["bitrate"], vbv_size=s["bitrate"], init_quant=10,
                        fixed_quant=s.get("fixed", 0))
-        eng.generate_input(sid, s["seed"], s["length"], sigma=sigma, motion=s["motion"])
+        n = s["length"] if frames is None else min(frames, s["length"])
+        eng.generate_input(sid, s["seed"], n, sigma=s.get("sigma", sigma), motion=s["motion"],
+                           local=s.get("local", False), scene_cut=s.get("scene_cut", 0))
         ids.append(sid)
     eng.sync()
     return ids
 
 
-def run_job(eng, ids, specs, steps, compact_every=60):
-    """K frame steps (graph replay), closing finished streams and re-packing
-    the MB space every compact_every steps.  Returns (seconds, frames, mbs)."""
+def run_job(eng, ids, specs, steps, start=0, compact_every=60, after_chunk=None):
+    """frame steps start..start+steps-1 (graph replay; eager with events when
+    the engine profiles), closing finished streams and re-packing the MB
+    space every compact_every steps.  Returns (seconds, frames, mbs)."""
     lengths = np.array([s["length"] for s in specs])
     nmb = np.array([(s["size"][0] // 16) * (s["size"][1] // 16) for s in specs])
-    done = 0
-    open_mask = np.ones(len(ids), bool)
+    done = start
+    end = start + steps
+    open_mask = lengths > start
     eng.sync()
     t0 = time.perf_counter()
-    while done < steps:
-        chunk = min(compact_every, steps - done)
+    while done < end:
+        chunk = min(compact_every, end - done)
         eng.run(chunk)
         done += chunk
+        if after_chunk is not None:
+            after_chunk(done)
         fin = open_mask & (lengths <= done)
-        if fin.any() and done < steps:
+        if fin.any() and done < end:
             eng.sync()
             for i in np.nonzero(fin)[0]:
                 eng.close(ids[i])
@@ -103,8 +144,24 @@ def run_job(eng, ids, specs, steps, compact_every=60):
             eng.compact()
     eng.sync()
     dt = time.perf_counter() - t0
-    frames = np.minimum(lengths, steps)
+    frames = np.minimum(lengths, end) - np.minimum(lengths, start)
     return dt, int(frames.sum()), int((frames * nmb).sum())
+
+
+def algorithmic(specs, t0, t1):
+    """algorithmic HBM bytes of the ME and MB kernels and byte-SADs of frame
+    steps t0..t1-1 (intra frames: no reference read, no search)"""
+    me = mb = sad = 0
+    for s in specs:
+        w, h = s["size"]
+        nmb, fsz = w * h // 256, w * h * 3 // 2
+        n = max(0, min(s["length"], t1) - t0)
+        intra = 1 if (t0 == 0 and n > 0) else 0
+        inter = n - intra
+        me += inter * ME_BYTES[nmb] + intra * (w * h + 16 * nmb)
+        mb += inter * MB_BYTES[nmb] + intra * (2 * fsz + nmb * (768 + 16))
+        sad += inter * SAD_OPS[nmb]
+    return me, mb, sad
 
 
 def plain_single_stream(Engine, frames=300, warmup=10):
@@ -132,22 +189,6 @@ def plain_single_stream(Engine, frames=300, warmup=10):
     return fps, float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
 
-def kernel_profile(Engine, specs, steps=12):
-    """per-kernel device time (HIP events on the engine stream around each
-    launch) with every stream active"""
-    eng = Engine(max_streams=len(specs))
-    ids = open_streams(eng, specs)
-    eng.step()  # the intra frame (no search) is not representative
-    eng.sync()
-    eng.profile(True)
-    for _ in range(steps):
-        eng.step()
-    eng.sync()
-    t = eng.kernel_times()
-    eng.destroy()
-    return t
-
-
 def copy_probe(nbytes):
     """one device-to-device copy of nbytes (the HIP runtime's copy kernel):
     the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE"""
@@ -163,7 +204,40 @@ def copy_probe(nbytes):
     hip.hipFree(b)
 
 
-def cpu_baseline(frames=30, workers=None):
+def hard_motion(Engine, steps=120, mbs_per_wave=0):
+    """the hard-motion batch over its first `steps` frame steps (the first
+    two untimed)"""
+    specs = hard_streams()
+    eng = Engine(max_streams=len(specs), mbs_per_wave=mbs_per_wave)
+    ids = open_streams(eng, specs, frames=steps)
+    eng.step()  # untimed: the intra frame, then one replay that captures the graph
+    eng.run(1)
+    dt, frames, mbs = run_job(eng, ids, specs, steps - 2, start=2, compact_every=steps)
+    eng.destroy()
+    return {"streams": len(specs), "steps": steps - 2, "frames": frames, "seconds": round(dt, 3),
+            "frames_per_s": round(frames / dt, 1), "cif_frames_per_s": round(mbs / 396.0 / dt, 1)}
+
+
+def serving(rate=2000.0, duration=0.5):
+    """Poisson arrivals (60% CIF / 40% QCIF, lognormal lengths, 64-1024
+    kbit/s, local-motion content), admission by HBM budget, continuous
+    batching"""
+    from h261enc_cuda_amd.serving import ServingLoop, poisson_requests
+    reqs = poisson_requests(rate=rate, duration=duration)
+    loop = ServingLoop(hbm_budget=96 << 30, max_streams=4096)
+    st = loop.run(reqs)
+    loop.close()
+    return {"rate_req_s": rate, "arrival_window_s": duration, "requests": st.requests, "frames": st.frames,
+            "seconds": round(st.seconds, 3), "frames_per_s": round(st.frames_per_s, 1),
+            "cif_frames_per_s": round(st.cif_frames_per_s, 1),
+            "frame_latency_ms_p50": round(st.frame_latency_ms_p50, 3),
+            "frame_latency_ms_p99": round(st.frame_latency_ms_p99, 3),
+            "request_latency_ms_p50": round(st.request_latency_ms_p50, 1),
+            "request_latency_ms_p99": round(st.request_latency_ms_p99, 1),
+            "max_concurrent_streams": st.max_concurrent, "steps": st.steps}
+
+
+def cpu_baseline(frames=150, workers=None):
     """The oracle (C restatement of the reference path) on the host cores:
     one CIF stream of the plain configuration per worker process."""
     import multiprocessing as mp
@@ -192,10 +266,12 @@ def _cpu_stream(seed, frames):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600, help="frame steps timed (600 = the whole job)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="frame steps timed after the warmup (default: the rest of the job, 600 - W)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-stream line")
+    ap.add_argument("--no-extra", action="store_true", help="skip the hard-motion and serving lines")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -203,77 +279,107 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(streams would shard across GPUs with no exchange)"}))
         sys.exit(2)
-    from h261enc_cuda_amd import Engine
+    from h261enc_cuda_amd import Engine, device_info
+    dev = device_info()
 
     if args.copy_probe:
         copy_probe(256 << 20)
 
     specs = batched_streams()
-    # warmup: a few steps of a small engine (module load, kernels, graph)
-    w = Engine(max_streams=4)
-    wids = open_streams(w, specs[:2] + specs[256:258])
-    for _ in range(args.warmup):
-        w.step()
-    w.run(args.warmup)
-    w.sync()
-    w.destroy()
+    job = max(s["length"] for s in specs)
+    W = max(0, min(args.warmup, job - 1))
+    K = job - W if args.steps is None else max(1, min(args.steps, job - W))
 
+    # pass A (the timed region): W untimed steps of the job, the last one a
+    # graph replay so capture happens before the clock starts, then K steps
     eng = Engine(max_streams=len(specs))
     ids = open_streams(eng, specs)
-    steps = min(args.steps, max(s["length"] for s in specs))
-    dt, frames, mbs = run_job(eng, ids, specs, steps)
+    for _ in range(W - 1):
+        eng.step()
+    if W:
+        eng.run(1)
+    eng.sync()
+    dt, frames, mbs = run_job(eng, ids, specs, K, start=W)
     eng.destroy()
     cif_equiv = mbs / 396.0
     value = cif_equiv / dt
 
-    prof = kernel_profile(Engine, specs)
-    n_cif, n_qcif = 256, 256
-    me_ms = prof["me_ms"] / prof["steps"]
-    mb_ms = prof["mb_ms"] / prof["steps"]
-    me_bytes = n_cif * ME_BYTES[396] + n_qcif * ME_BYTES[99]
-    mb_bytes = n_cif * MB_BYTES[396] + n_qcif * MB_BYTES[99]
-    sad_ops = n_cif * SAD_OPS[396] + n_qcif * SAD_OPS[99]
-    me_gbs = me_bytes / (me_ms * 1e-3) / 1e9
-    mb_gbs = mb_bytes / (mb_ms * 1e-3) / 1e9
-    sad_rate = sad_ops / (me_ms * 1e-3)
-    dominant = "mb_kernel" if mb_ms >= me_ms else "me_kernel"
-    if dominant == "mb_kernel":
-        roof = {"kernel": "mb_kernel", "bound": "hbm", "achieved": round(mb_gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(mb_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_ms": round(mb_ms, 4), "algorithmic_bytes_per_launch": mb_bytes}
-    else:
-        roof = {"kernel": "me_kernel", "bound": "hbm", "achieved": round(me_gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(me_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_ms": round(me_ms, 4), "algorithmic_bytes_per_launch": me_bytes}
-    roof["me_kernel"] = {"avg_launch_ms": round(me_ms, 4), "hbm_gbs": round(me_gbs, 1),
-                         "sad_rate_tops": round(sad_rate / 1e12, 3), "sad_frac": round(sad_rate / SAD_PEAK, 4)}
-    roof["mb_kernel"] = {"avg_launch_ms": round(mb_ms, 4), "hbm_gbs": round(mb_gbs, 1),
-                         "hbm_frac": round(mb_gbs / HBM_PEAK_GBS, 4)}
-    roof["other_ms_per_step"] = round(prof["other_ms"] / prof["steps"], 4)
+    # pass B: the same W + K steps again, each kernel bracketed by HIP events
+    # on the engine stream; the averages are over the same launches as pass A
+    eng = Engine(max_streams=len(specs))
+    ids = open_streams(eng, specs)
+    for _ in range(W):
+        eng.step()
+    eng.sync()
+    eng.profile(True)
+    first = {}
+
+    def after_chunk(done):
+        if not first:
+            first.update(eng.kernel_times(), done=done)
+
+    run_job(eng, ids, specs, K, start=W, after_chunk=after_chunk)
+    prof = eng.kernel_times()
+    eng.destroy()
+
+    def kernel_figures(t, t0, t1):
+        n = t["steps"]
+        me_b, mb_b, sad = algorithmic(specs, t0, t1)
+        me_ms, mb_ms = t["me_ms"] / n, t["mb_ms"] / n
+        return dict(launches=n, me_ms=me_ms, mb_ms=mb_ms, me_bytes=me_b / n, mb_bytes=mb_b / n,
+                    me_gbs=me_b / n / (me_ms * 1e-3) / 1e9, mb_gbs=mb_b / n / (mb_ms * 1e-3) / 1e9,
+                    sad_rate=sad / n / (me_ms * 1e-3), other_ms=t["other_ms"] / n)
+
+    job_k = kernel_figures(prof, W, W + K)
+    full_k = kernel_figures(first, W, first["done"])
+    dominant = "mb_kernel" if job_k["mb_ms"] >= job_k["me_ms"] else "me_kernel"
+    key = "mb" if dominant == "mb_kernel" else "me"
+    achieved = job_k[key + "_gbs"]
+    roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_per_launch(dominant, job_k[key + "_bytes"]),
+            "avg_launch_ms": round(job_k[key + "_ms"], 4),
+            "algorithmic_bytes_per_launch": int(job_k[key + "_bytes"]), "launches": job_k["launches"]}
+    for name, f in (("job_average", job_k), ("full_load", full_k)):
+        roof[name] = {"launches": f["launches"],
+                      "me_kernel": {"avg_launch_ms": round(f["me_ms"], 4), "hbm_gbs": round(f["me_gbs"], 1),
+                                    "sad_rate_tops": round(f["sad_rate"] / 1e12, 3),
+                                    "sad_frac": round(f["sad_rate"] / SAD_PEAK, 4)},
+                      "mb_kernel": {"avg_launch_ms": round(f["mb_ms"], 4), "hbm_gbs": round(f["mb_gbs"], 1),
+                                    "hbm_frac": round(f["mb_gbs"] / HBM_PEAK_GBS, 4)},
+                      "other_ms_per_step": round(f["other_ms"], 4)}
 
     plain = None
     if not args.no_plain:
         fps, p50, p99 = plain_single_stream(Engine)
         plain = {"fps": round(fps, 1), "latency_ms_p50": round(p50, 3), "latency_ms_p99": round(p99, 3)}
 
+    hard = serve = None
+    if not args.no_extra:
+        hard = hard_motion(Engine)
+        serve = serving()
+
     cpu = None
     if not args.no_cpu:
         cfps, cores, cdt, cpu_s = cpu_baseline()
         cpu = {"value": round(cfps, 2), "unit": "CIF frames/s", "cores": cores, "kind": "port",
-               "sample": f"{cores} CIF streams x 30 frames (fixed QUANT 8, the plain configuration's content), "
+               "sample": f"{cores} CIF streams x 150 frames (fixed QUANT 8, the plain configuration's content), "
                          f"one per process, {cdt:.1f} s wall, {cpu_s:.1f} s CPU"}
 
     line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "CIF frames/s", "n_gpus": 1, "steps": steps,
-        "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / steps, 4), "higher_is_better": True,
+        "metric": METRIC, "value": round(value, 1), "unit": "CIF frames/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": round(dt * 1e3 / K, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": "batched: 256 CIF + 256 QCIF streams, lengths U{60..600}, global motion +-15 px, "
-                               "rate control 64-384 kbit/s, inputs resident in HBM, whole job",
+                               "rate control 64-384 kbit/s, inputs resident in HBM; timed: the job's frame steps after "
+                               "the warmup to its end",
                    "frames": frames, "cif_equivalent_frames": round(cif_equiv, 1),
                    "frames_per_s": round(frames / dt, 1), "seconds": round(dt, 3),
                    "plain_single_cif": plain,
                    "batched_over_plain": round(value / plain["fps"], 2) if plain else None,
-                   "parallelism": "1 GPU, all streams per frame step"},
+                   "hard_motion_batch": hard, "serving": serve,
+                   "parallelism": "1 GPU, all streams per frame step",
+                   "device": f"{dev['name']} {dev['arch']}, {dev['compute_units']} CUs, {dev['clock_mhz']} MHz"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

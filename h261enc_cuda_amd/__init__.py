@@ -5,6 +5,6 @@ package is its Python host side: the ctypes binding (_native), the
 multi-stream engine wrapper and the synthetic content generator (encoder).
 """
 from . import _native  # noqa: F401
-from .encoder import Engine, LegacyDevice, content_frame, write_picture  # noqa: F401
+from .encoder import Engine, LegacyDevice, content_frame, device_info, write_picture  # noqa: F401
 
-__all__ = ["Engine", "LegacyDevice", "content_frame", "write_picture"]
+__all__ = ["Engine", "LegacyDevice", "content_frame", "device_info", "write_picture"]

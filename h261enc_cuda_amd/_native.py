@@ -50,11 +50,12 @@ EXPORTS = [
     "h261g_engine_sync", "h261g_stream_info", "h261g_stream_read_macroblocks", "h261g_stream_read_recon",
     "h261g_stream_read_me", "h261g_stream_read_bitstream", "h261g_stream_read_input",
     "h261g_engine_profile", "h261g_engine_kernel_times", "h261g_content_frame_host", "h261g_write_picture",
-    "h261g_engine_config",
+    "h261g_engine_config", "h261g_device_info",
 ]
 OPT_ME_VARIANT = 1
 OPT_MBS_PER_WAVE = 2
 OPT_FUSED_TAILS = 3
+OPT_XCD_REMAP = 4
 
 
 class StreamParams(ctypes.Structure):
@@ -66,6 +67,11 @@ class StreamParams(ctypes.Structure):
 class ContentParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("sigma", ctypes.c_int32), ("motion", ctypes.c_int32),
                 ("local", ctypes.c_int32), ("scene_cut", ctypes.c_int32)]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("compute_units", ctypes.c_int32), ("clock_khz", ctypes.c_int32), ("l2_bytes", ctypes.c_int64),
+                ("memory_bytes", ctypes.c_int64), ("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 64)]
 
 
 class FrameInfo(ctypes.Structure):
@@ -94,6 +100,7 @@ def load() -> ctypes.CDLL:
         "h261g_engine_create": (ctypes.c_int, [ctypes.POINTER(P), i32, i64, i32]),
         "h261g_engine_destroy": (ctypes.c_int, [P]),
         "h261g_engine_config": (ctypes.c_int, [P, i32, i32]),
+        "h261g_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
         "h261g_stream_footprint": (i64, [i32, i32]),
         "h261g_stream_open": (ctypes.c_int, [P, ctypes.POINTER(StreamParams)]),
         "h261g_stream_close": (ctypes.c_int, [P, i32]),

@@ -5,15 +5,19 @@
  * device-generated frame sequence), two reconstruction buffers used as
  * reference / reconstruction in turn, and its DenoiseState and RcState.
  * All streams' macroblocks sit in one global MB index space (mb_base per
- * stream, a map from global MB to stream), so one frame step is six
+ * stream, a map from global MB to stream), so one frame step is three
  * launches over all streams at once:
  *
- *   me_kernel          search + SATD costs + mode decision, cost sum
- *   rc_begin_kernel    picture cost, rate control, picture quantiser
- *   mb_kernel          the fused macroblock coder, bit counts
- *   denoise_update     offsets for the next frame
- *   frame_bits_kernel  exact frame size (GOB scan), rate-control update
- *   advance_kernel     swap reference/reconstruction, next input frame
+ *   me_kernel          search + SATD costs + mode decision; the MB that
+ *                      completes its stream runs rate-control begin
+ *                      (picture cost, rate control, picture quantiser)
+ *   mb_kernel          the fused macroblock coder, bit counts, statistics
+ *   frame_tail_kernel  one wave per stream: denoise offsets for the next
+ *                      frame, exact frame size (GOB scan), rate-control
+ *                      update, swap reference/reconstruction, next input
+ *
+ * (H261G_OPT_FUSED_TAILS 0: the same work as six launches: me_kernel,
+ * rc_begin, mb_kernel, denoise_update, frame_bits, advance.)
  *
  * Nothing in a step depends on the host, so h261g_engine_run captures the
  * step once in a HIP graph and replays it.
@@ -36,15 +40,6 @@ int generate_frames_device(const h261g_content_params_t &c, int width, int heigh
                            void *stream);
 
 namespace {
-
-__global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    StreamDesc &d = streams[s];
-    if (!(d.flags & kStreamActive)) return;
-    fc::advance_one(d, seq[s]);
-}
 
 bool ok(hipError_t e, const char *what)
 {
@@ -78,6 +73,7 @@ struct h261g_engine {
     int mbs_per_wave = 1;
     int me_variant = kMeTwoStage;
     int fused_tails = 1;
+    int xcd_remap = 0;
     hipStream_t stream = nullptr;
     std::vector<Slot> slots;
     /* device arrays indexed by stream slot */
@@ -101,7 +97,7 @@ struct h261g_engine {
     bool graph_valid = false;
     /* profiling */
     bool profile = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double me_ms = 0, mb_ms = 0, other_ms = 0;
     int64_t launches = 0;
 };
@@ -184,6 +180,7 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.skip_search_on_intra = 1;
     ml.variant = e->me_variant;
     ml.fuse_tail = fused;
+    ml.xcd_remap = e->xcd_remap;
     if (launch_me(ml, st)) return H261G_ERR_DEVICE;
     if (timed) (void)hipEventRecord(e->ev[1], st);
     if (!fused && launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
@@ -198,28 +195,30 @@ int enqueue_step(h261g_engine *e, bool timed)
     bl.side = e->side;
     bl.levels = e->levels;
     bl.mbs_per_wave = e->mbs_per_wave;
-    bl.fuse_tail = fused;
-    bl.seq = e->seq;
+    bl.xcd_remap = e->xcd_remap;
     if (launch_mb(bl, st)) return H261G_ERR_DEVICE;
     if (timed) (void)hipEventRecord(e->ev[3], st);
-    if (!fused) {
+    if (fused) {
+        if (launch_frame_tail(e->desc, e->seq, e->dn, e->rc, e->side, ns, st)) return H261G_ERR_DEVICE;
+    } else {
         if (launch_denoise_update(e->dn, e->desc, ns, st)) return H261G_ERR_DEVICE;
         if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st)) return H261G_ERR_DEVICE;
-        hipLaunchKernelGGL(advance_kernel, dim3((ns + 63) / 64), dim3(64), 0, st, e->desc, e->seq, ns);
-        if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
+        if (launch_advance(e->desc, e->seq, ns, st)) return H261G_ERR_DEVICE;
     }
+    if (timed) (void)hipEventRecord(e->ev[4], st);
     return H261G_OK;
 }
 
 int accumulate_times(h261g_engine *e)
 {
-    if (!ok(hipEventSynchronize(e->ev[3]), "event")) return H261G_ERR_DEVICE;
-    float a = 0, b = 0, c = 0;
+    if (!ok(hipEventSynchronize(e->ev[4]), "event")) return H261G_ERR_DEVICE;
+    float a = 0, b = 0, c = 0, f = 0;
     (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
     (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
     (void)hipEventElapsedTime(&c, e->ev[2], e->ev[3]);
+    (void)hipEventElapsedTime(&f, e->ev[3], e->ev[4]);
     e->me_ms += a;
-    e->other_ms += b;
+    e->other_ms += b + f;
     e->mb_ms += c;
     e->launches++;
     return H261G_OK;
@@ -251,10 +250,31 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         if (value != 0 && value != 1) return H261G_ERR_ARG;
         e->fused_tails = value;
         break;
+    case H261G_OPT_XCD_REMAP:
+        if (value != 0 && value != 1) return H261G_ERR_ARG;
+        e->xcd_remap = value;
+        break;
     default:
         return H261G_ERR_ARG;
     }
     e->graph_valid = false;
+    return H261G_OK;
+}
+
+int h261g_device_info(h261g_device_info_t *out)
+{
+    if (!out) return H261G_ERR_ARG;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (!ok(hipGetDevice(&dev), "hipGetDevice") || !ok(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties"))
+        return H261G_ERR_DEVICE;
+    *out = h261g_device_info_t{};
+    out->compute_units = prop.multiProcessorCount;
+    out->clock_khz = prop.clockRate;
+    out->l2_bytes = prop.l2CacheSize;
+    out->memory_bytes = (int64_t)prop.totalGlobalMem;
+    snprintf(out->name, sizeof(out->name), "%s", prop.name);
+    snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
     return H261G_OK;
 }
 
@@ -282,7 +302,7 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipMalloc((void **)&e->rc, sizeof(RcState) * max_streams), "hipMalloc") &&
                 ok(hipMemset(e->desc, 0, sizeof(StreamDesc) * max_streams), "hipMemset") &&
                 ok(hipMemset(e->seq, 0, sizeof(SeqState) * max_streams), "hipMemset");
-    for (int i = 0; good && i < 4; i++) good = ok(hipEventCreate(&e->ev[i]), "event");
+    for (int i = 0; good && i < 5; i++) good = ok(hipEventCreate(&e->ev[i]), "event");
     if (!good) {
         h261g_engine_destroy(e);
         return H261G_ERR_DEVICE;
@@ -312,7 +332,7 @@ int h261g_engine_destroy(h261g_engine_t *e)
     hipFree(e->side);
     hipFree(e->levels);
     hipHostFree(e->mbstruct);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 5; i++)
         if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;

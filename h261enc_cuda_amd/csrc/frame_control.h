@@ -1,7 +1,7 @@
 /*
  * frame_control.h -- per-stream frame control as device functions, shared by
- * the standalone kernels (frame_kernel.hip) and the fused tails of the
- * motion-estimation and macroblock kernels:
+ * the separate launches and the fused forms (frame_kernel.hip, the tail of
+ * the motion-estimation kernel):
  *
  *   rc_begin_one        picture cost, h261_ratectrl_init_picture, the
  *                       fixed-quant override, h261_init_picture's clamp
@@ -17,9 +17,16 @@
  *                       (MQUANT state) and the MV predictor (MB i-1, reset at
  *                       MBs 1, 12, 23) come from ballots and lane shuffles.
  *   rc_update_one       h261_ratectrl_update_picture (ratectrl.c:198-222)
- *   denoise_update_wave macroblocks_update_dct_denoise (mbgpu:478-503)
+ *   denoise_update_lane macroblocks_update_dct_denoise (mbgpu:478-503)
  *   advance_one         the reconstruction becomes the next reference, the
  *                       next input frame is selected
+ *   frame_tail_wave     the four above that end a frame, in one wave
+ *
+ * Fusing stays fence-free: an agent-scope fence on this device writes back
+ * and invalidates the XCD's L2, which costs far more than a launch when
+ * every wave issues one.  The only fused piece inside a data-parallel
+ * kernel is rate-control begin, whose single input (the frame's cost sum)
+ * travels in the same 64-bit atomic as the completion count.
  */
 #ifndef H261G_FRAME_CONTROL_H
 #define H261G_FRAME_CONTROL_H
@@ -76,11 +83,11 @@ __device__ __forceinline__ int rc_quant_for_qscale(int qs)
     return pos - 31;
 }
 
-__device__ __forceinline__ void rc_begin_one(RcState &rc, const StreamDesc &d)
+/* cost_sum: the MD costs of the frame's MBs summed */
+__device__ __forceinline__ void rc_begin_one(RcState &rc, const StreamDesc &d, int cost_sum)
 {
     const int num_mb = d.mb_w * d.mb_h;
-    rc.picture_cost = imax(rc.cost_accum, num_mb * 100);
-    rc.cost_accum = 0;
+    rc.picture_cost = imax(cost_sum, num_mb * 100);
     const int ft = d.frame_type;
     rc.vbv_fill += rc_div(rc.bitrate, kRcTimeScale);
     rc.vbv_fill = imin(rc.vbv_fill, rc.vbv_size);
@@ -205,6 +212,28 @@ __device__ __forceinline__ void advance_one(StreamDesc &d, SeqState &q)
         }
     } else {
         d.flags &= ~kStreamActive; /* a host-fed stream waits for its next frame */
+    }
+}
+
+/* the end of one stream's frame, all 64 lanes of one wave: denoise offsets
+ * for the next frame, exact frame size, rate-control update, advance.  Reads
+ * only what earlier launches wrote (statistics, side information). */
+__device__ __forceinline__ void frame_tail_wave(StreamDesc &d, SeqState &q, DenoiseState *dn, RcState &rc,
+                                                const MbSide *side, int32_t *bytes_out, int lane)
+{
+    const int ci = dn->inter_count, ca = dn->intra_count;
+    denoise_update_lane(dn, lane, ci, ca);
+    const int bits = frame_bits_wave(d, side, rc.pic_quant, lane);
+    __syncthreads();
+    if (lane == 0) {
+        dn->inter_count = 0;
+        dn->intra_count = 0;
+        const int bytes = (bits + 7) >> 3;
+        rc.frame_bits = bits;
+        rc.frame_bytes = bytes;
+        rc_update_one(rc, d.frame_type, bytes);
+        if (bytes_out) *bytes_out = bytes;
+        advance_one(d, q);
     }
 }
 

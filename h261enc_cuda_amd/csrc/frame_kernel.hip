@@ -1,8 +1,8 @@
 /*
- * frame_kernel.hip -- per-stream frame control as standalone launches (the
- * engine's unfused mode, and the legacy path's denoise update).  The same
- * device functions (frame_control.h) run as fused tails inside the motion
- * estimation and macroblock kernels in the engine's default mode.
+ * frame_kernel.hip -- per-stream frame control launches: the fused frame
+ * tail (denoise update, frame size, rate-control update, advance: one wave
+ * per stream) of the engine's default mode, and the separate kernels of its
+ * unfused mode and of the legacy path's denoise update.
  */
 #include <hip/hip_runtime.h>
 
@@ -19,7 +19,9 @@ __global__ void rc_begin_kernel(RcState *rc_all, const StreamDesc *streams, int 
     if (s >= n) return;
     const StreamDesc &d = streams[s];
     if (!(d.flags & kStreamActive)) return;
-    fc::rc_begin_one(rc_all[s], d);
+    RcState &rc = rc_all[s];
+    fc::rc_begin_one(rc, d, rc.cost_accum);
+    rc.cost_accum = 0;
 }
 
 __global__ __launch_bounds__(64) void frame_bits_kernel(RcState *rc_all, const StreamDesc *streams, int n,
@@ -53,7 +55,42 @@ __global__ __launch_bounds__(64) void denoise_update_kernel(DenoiseState *dn_all
     }
 }
 
+__global__ __launch_bounds__(64) void frame_tail_kernel(StreamDesc *streams, SeqState *seq, DenoiseState *dn_all,
+                                                         RcState *rc_all, const MbSide *side, int32_t *bytes_out)
+{
+    const int s = blockIdx.x;
+    StreamDesc &d = streams[s];
+    if (!(d.flags & kStreamActive)) return;
+    fc::frame_tail_wave(d, seq[s], dn_all + s, rc_all[s], side, bytes_out ? bytes_out + s : nullptr, threadIdx.x);
+}
+
+__global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    StreamDesc &d = streams[s];
+    if (!(d.flags & kStreamActive)) return;
+    fc::advance_one(d, seq[s]);
+}
+
 }  // namespace
+
+int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
+                      int n_streams, void *stream)
+{
+    if (n_streams <= 0) return 0;
+    hipLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, streams, seq, dn, rc,
+                       side, nullptr);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_advance(StreamDesc *streams, SeqState *seq, int n_streams, void *stream)
+{
+    if (n_streams <= 0) return 0;
+    hipLaunchKernelGGL(advance_kernel, dim3((n_streams + 63) / 64), dim3(64), 0, (hipStream_t)stream, streams, seq,
+                       n_streams);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int, void *stream)
 {

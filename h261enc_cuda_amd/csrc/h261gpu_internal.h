@@ -99,13 +99,32 @@ struct RcState {
     int32_t bitrate, vbv_size, vbv_fill;
     int32_t fixed_quant; /* > 0: picture quantiser override */
     int32_t pic_quant;   /* quantiser chosen for the current frame */
-    int32_t cost_accum;  /* sum of MD costs of the current frame */
+    int32_t cost_accum;  /* sum of MD costs of the current frame (separate rate-control launch) */
     int32_t frame_bytes; /* bytes of the last coded frame */
     int32_t frame_bits;
-    int32_t me_done;     /* MBs of this step through motion estimation (fused tail) */
-    int32_t mb_done;     /* MBs of this step through the macroblock coder (fused tail) */
     int32_t pad;
+    /* fused rate-control begin: MBs through motion estimation (high 32 bits)
+     * and the sum of their MD costs (low 32 bits), one 64-bit atomic per MB,
+     * so the MB that completes the count also holds the frame's total cost
+     * and no memory fence is needed */
+    uint64_t me_acc;
 };
+static_assert(sizeof(RcState) == 72, "RcState layout");
+
+/* Workgroups are dispatched to the 8 XCDs of the device in turn, each XCD
+ * with its own L2.  Neighbouring macroblocks share reference and search
+ * window lines, so a launch of n logical workgroups is padded to a multiple
+ * of 8 and workgroup i is given logical index (i % 8) * per + i / 8: every
+ * XCD codes a contiguous run of macroblocks.  Indexes >= n exit. */
+constexpr int kXcds = 8;
+H261G_HD inline int xcd_grid(int n) { return ((n + kXcds - 1) / kXcds) * kXcds; }
+H261G_HD inline int xcd_logical(int i, int n)
+{
+    const int per = (n + kXcds - 1) / kXcds;
+    return (i % kXcds) * per + i / kXcds;
+}
+H261G_HD inline int launch_grid(int n, bool remap) { return remap ? xcd_grid(n) : n; }
+H261G_HD inline int launch_index(int i, int n, bool remap) { return remap ? xcd_logical(i, n) : i; }
 
 /* ---- launchers (implemented in the .hip files) ---- */
 
@@ -122,7 +141,8 @@ struct MeLaunch {
     RcState *rc;        /* optional per-stream: cost accumulation */
     int32_t skip_search_on_intra; /* 1: intra frames only compute the intra cost */
     int32_t variant;              /* kMeTwoStage or kMeSurface */
-    int32_t fuse_tail;            /* 1: the last MB of a stream runs rate-control begin */
+    int32_t fuse_tail;            /* 1: the MB that completes a stream runs rate-control begin */
+    int32_t xcd_remap;            /* 1: contiguous macroblock runs per XCD (xcd_logical) */
 };
 enum : int32_t {
     kMeTwoStage = 0, /* the reference's candidates only, one v_sad_u8 lane per candidate */
@@ -133,7 +153,7 @@ int launch_me(const MeLaunch &p, void *stream);
 /* one wave per MB: MC/loop filter, DCT, denoise, quant, dequant, IDCT,
  * reconstruction; statistics for the denoise update; bit counts. */
 struct MbLaunch {
-    StreamDesc *streams; /* written by the fused tail (advance) */
+    const StreamDesc *streams;
     const int32_t *mb_stream;
     int32_t n_mb;
     const MbMode *mode;           /* [n_mb] */
@@ -143,9 +163,7 @@ struct MbLaunch {
     int16_t *levels;              /* optional [n_mb][6][64] */
     h261_macroblock_t *mb_struct; /* optional [n_mb] legacy layout */
     int32_t mbs_per_wave;         /* MBs coded by one wave in sequence */
-    int32_t fuse_tail;            /* 1: the last wave of a stream runs the denoise update,
-                                     frame size, rate-control update and advance */
-    SeqState *seq;                /* per stream, with fuse_tail */
+    int32_t xcd_remap;            /* 1: contiguous macroblock runs per XCD (xcd_logical) */
 };
 int launch_mb(const MbLaunch &p, void *stream);
 
@@ -160,6 +178,12 @@ int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int n
  * size, rate-control update */
 int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, const MbSide *side,
                       int32_t *frame_bytes_out, void *stream);
+/* per stream: the reconstruction becomes the reference, next input frame */
+int launch_advance(StreamDesc *streams, SeqState *seq, int n_streams, void *stream);
+/* per stream, one wave: denoise update, frame size, rate-control update and
+ * advance in one launch */
+int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
+                      int n_streams, void *stream);
 
 }  // namespace h261g
 

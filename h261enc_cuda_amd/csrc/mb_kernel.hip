@@ -25,7 +25,6 @@
  */
 #include <hip/hip_runtime.h>
 
-#include "frame_control.h"
 #include "h261_vlc.h"
 #include "h261gpu_internal.h"
 
@@ -129,34 +128,9 @@ __device__ __forceinline__ int tcoeff_bits(int run, int level)
     return kEscapeBits;
 }
 
-/* the end of a stream of a frame step, run by the wave that codes its last
- * MB: denoise update, exact frame size, rate-control update, advance */
-__device__ __forceinline__ void frame_tail(const MbLaunch &p, int s, int lane)
-{
-    StreamDesc &d = p.streams[s];
-    DenoiseState *dn = p.denoise + s;
-    const int ci = dn->inter_count, ca = dn->intra_count;
-    fc::denoise_update_lane(dn, lane, ci, ca);
-    __syncthreads();
-    RcState &rc = p.rc[s];
-    const int bits = fc::frame_bits_wave(d, p.side, rc.pic_quant, lane);
-    if (lane == 0) {
-        dn->inter_count = 0;
-        dn->intra_count = 0;
-        const int bytes = (bits + 7) >> 3;
-        rc.frame_bits = bits;
-        rc.frame_bytes = bytes;
-        fc::rc_update_one(rc, d.frame_type, bytes);
-        rc.mb_done = 0;
-        fc::advance_one(d, p.seq[s]);
-    }
-}
-
 /* flushes this wave's denoise statistics of stream s (LDS -> global
- * atomics) and, with fused tails, counts its MBs; the wave that completes
- * the stream's MB count runs the frame tail */
-__device__ __forceinline__ void flush_segment(const MbLaunch &p, int s, int count, int32_t (*stat)[64], int32_t *scount,
-                                              int lane)
+ * atomics); the frame tail reads them in a later launch */
+__device__ __forceinline__ void flush_segment(const MbLaunch &p, int s, int32_t (*stat)[64], int32_t *scount, int lane)
 {
     __syncthreads();
     DenoiseState *dn = p.denoise + s;
@@ -168,20 +142,6 @@ __device__ __forceinline__ void flush_segment(const MbLaunch &p, int s, int coun
     stat[0][lane] = 0;
     stat[1][lane] = 0;
     if (lane < 2) scount[lane] = 0;
-    if (p.fuse_tail) {
-        /* every lane's stores (levels, side, reconstruction) and statistic
-         * atomics are visible device-wide before the count */
-        __threadfence();
-        __syncthreads();
-        int done = 0;
-        if (lane == 0) done = atomicAdd(&p.rc[s].mb_done, count);
-        done = __shfl(done, 0, 64);
-        const StreamDesc &d = p.streams[s];
-        if (done + count == d.mb_w * d.mb_h) {
-            __threadfence(); /* acquire: the other waves' results */
-            frame_tail(p, s, lane);
-        }
-    }
     __syncthreads();
 }
 
@@ -201,16 +161,17 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     stat[0][lane] = 0;
     stat[1][lane] = 0;
     if (lane < 2) scount[lane] = 0;
-    int seg_count = 0; /* MBs of the current stream coded by this wave since the last flush */
 
+    const int n_wg = (p.n_mb + p.mbs_per_wave - 1) / p.mbs_per_wave;
+    const int wg = launch_index(blockIdx.x, n_wg, p.xcd_remap);
+    if (wg >= n_wg) return;
     for (int it = 0; it < p.mbs_per_wave; it++) {
-        const int g = blockIdx.x * p.mbs_per_wave + it;
+        const int g = wg * p.mbs_per_wave + it;
         if (g >= p.n_mb) break;
         const int s = p.mb_stream[g];
         if (s < 0) continue; /* unmapped range of a closed stream */
         const StreamDesc &d = p.streams[s];
         if (!(d.flags & kStreamActive)) continue;
-        seg_count++;
         const int m = g - d.mb_base;
         const MbMode md = p.mode[g];
         const int W = d.width;
@@ -520,10 +481,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
          * MB belongs to another stream (or there is none) */
         const int gn = g + 1;
         const bool last = it + 1 == p.mbs_per_wave || gn >= p.n_mb || p.mb_stream[gn] != s;
-        if (last) {
-            flush_segment(p, s, seg_count, stat, scount, lane);
-            seg_count = 0;
-        }
+        if (last) flush_segment(p, s, stat, scount, lane);
     }
     (void)sblk;
 }
@@ -536,7 +494,7 @@ int launch_mb(const MbLaunch &p, void *stream)
     const int per = p.mbs_per_wave > 0 ? p.mbs_per_wave : 1;
     MbLaunch q = p;
     q.mbs_per_wave = per;
-    const int grid = (p.n_mb + per - 1) / per;
+    const int grid = launch_grid((p.n_mb + per - 1) / per, p.xcd_remap);
     hipLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, q);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

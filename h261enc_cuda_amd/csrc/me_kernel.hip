@@ -159,7 +159,8 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
     __shared__ __align__(16) uint32_t curw[64];
     __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
 
-    const int g = blockIdx.x;
+    const int g = launch_index(blockIdx.x, p.n_mb, p.xcd_remap);
+    if (g >= p.n_mb) return;
     const int lane = threadIdx.x;
     const int s = p.mb_stream[g];
     if (s < 0) return; /* unmapped range of a closed stream */
@@ -313,17 +314,19 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
             }
             p.mode[g] = md;
             if (p.rc) {
-                atomicAdd(&p.rc[s].cost_accum, md.cost);
                 if (p.fuse_tail) {
-                    /* the last MB of the stream through ME starts its picture:
-                     * picture cost, rate control, quantiser */
-                    __threadfence();
-                    const int done = atomicAdd(&p.rc[s].me_done, 1);
-                    if (done == d.mb_w * d.mb_h - 1) {
-                        __threadfence();
-                        fc::rc_begin_one(p.rc[s], d);
-                        p.rc[s].me_done = 0;
+                    /* count and cost sum in one atomic: the MB that completes
+                     * the count holds the frame's cost sum and starts the
+                     * picture (rate control, quantiser) */
+                    RcState &rc = p.rc[s];
+                    const unsigned long long old =
+                        atomicAdd((unsigned long long *)&rc.me_acc, (1ull << 32) | (uint32_t)md.cost);
+                    if ((int)(old >> 32) == d.mb_w * d.mb_h - 1) {
+                        fc::rc_begin_one(rc, d, (int)(uint32_t)old + md.cost);
+                        atomicExch((unsigned long long *)&rc.me_acc, 0ull);
                     }
+                } else {
+                    atomicAdd(&p.rc[s].cost_accum, md.cost);
                 }
             }
         }
@@ -336,9 +339,9 @@ int launch_me(const MeLaunch &p, void *stream)
 {
     if (p.n_mb <= 0) return 0;
     if (p.variant == kMeSurface)
-        hipLaunchKernelGGL(me_kernel<kMeSurface>, dim3(p.n_mb), dim3(64), 0, (hipStream_t)stream, p);
+        hipLaunchKernelGGL(me_kernel<kMeSurface>, dim3(launch_grid(p.n_mb, p.xcd_remap)), dim3(64), 0, (hipStream_t)stream, p);
     else
-        hipLaunchKernelGGL(me_kernel<kMeTwoStage>, dim3(p.n_mb), dim3(64), 0, (hipStream_t)stream, p);
+        hipLaunchKernelGGL(me_kernel<kMeTwoStage>, dim3(launch_grid(p.n_mb, p.xcd_remap)), dim3(64), 0, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -43,6 +43,16 @@ def write_picture(mbs: np.ndarray, width: int, height: int, frame_num: int, quan
     return out[:n].tobytes()
 
 
+def device_info() -> dict:
+    """the current HIP device: compute units, clock, L2, memory, name, arch"""
+    lib = nat.load()
+    d = nat.DeviceInfo()
+    check(lib.h261g_device_info(ctypes.byref(d)), "device_info")
+    return {"compute_units": d.compute_units, "clock_mhz": d.clock_khz // 1000, "l2_bytes": d.l2_bytes,
+            "memory_gib": round(d.memory_bytes / 2 ** 30, 1), "name": d.name.decode(errors="replace"),
+            "arch": d.arch.decode(errors="replace")}
+
+
 class Engine:
     """Many independent QCIF/CIF streams, one frame step at a time."""
 

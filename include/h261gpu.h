@@ -121,7 +121,20 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
 #define H261G_OPT_MBS_PER_WAVE 2 /* macroblocks one wave of the macroblock coder codes */
 #define H261G_OPT_FUSED_TAILS 3  /* 1 (default): per-stream frame control runs in the tails of the
                                     two frame-step kernels; 0: as four separate launches */
+#define H261G_OPT_XCD_REMAP 4    /* 1: each XCD codes a contiguous run of macroblocks;
+                                    0 (default): workgroup i codes macroblock i */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
+
+/* the device the library runs on (the current HIP device) */
+typedef struct {
+    int32_t compute_units;
+    int32_t clock_khz;
+    int64_t l2_bytes;
+    int64_t memory_bytes;
+    char name[64];
+    char arch[64];
+} h261g_device_info_t;
+H261G_API int h261g_device_info(h261g_device_info_t *out);
 
 /* device bytes one stream of the given size occupies */
 H261G_API int64_t h261g_stream_footprint(int32_t width, int32_t height);

@@ -50,3 +50,19 @@ def test_product_does_not_reference_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")) or f == "Makefile":
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower(), f"{f} mentions the oracle"
+
+
+def test_xcd_remap_is_a_bijection():
+    """The XCD-aware workgroup order (h261gpu_internal.h xcd_logical) covers
+    every logical index exactly once for any launch size."""
+    def grid(n):
+        return ((n + 7) // 8) * 8
+
+    def logical(i, n):
+        per = (n + 7) // 8
+        return (i % 8) * per + i // 8
+
+    for n in list(range(1, 200)) + [396, 99, 126720, 63360 + 5]:
+        seen = [logical(i, n) for i in range(grid(n))]
+        inside = sorted(g for g in seen if g < n)
+        assert inside == list(range(n)), n
