@@ -97,7 +97,12 @@ struct h261g_engine {
     bool graph_valid = false;
     /* profiling */
     bool profile = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    /* profiling: a ring of event sets, one per step, read back lazily so the
+     * host keeps enqueueing ahead of the device and no host gap lands
+     * between an event and the kernel it brackets */
+    static constexpr int kEvRing = 64;
+    hipEvent_t ev[kEvRing][5] = {};
+    int64_t ev_recorded = 0, ev_read = 0;
     double me_ms = 0, mb_ms = 0, other_ms = 0;
     int64_t launches = 0;
 };
@@ -161,15 +166,21 @@ void invalidate_graph(h261g_engine *e)
 
 int stream_count(const h261g_engine *e) { return e->max_streams; }
 
-/* one frame step: two launches with the per-stream work fused into their
- * tails (default), or the six standalone launches */
+int read_oldest_times(h261g_engine *e);
+
+/* one frame step: three launches (default), or the six standalone launches */
 int enqueue_step(h261g_engine *e, bool timed)
 {
+    if (timed && e->ev_recorded - e->ev_read == h261g_engine::kEvRing) {
+        int rcode = read_oldest_times(e);
+        if (rcode) return rcode;
+    }
+    hipEvent_t *ev = e->ev[e->ev_recorded % h261g_engine::kEvRing];
     const int ns = stream_count(e);
     const int n_mb = e->mb_used;
     hipStream_t st = e->stream;
     const int fused = e->fused_tails;
-    if (timed) (void)hipEventRecord(e->ev[0], st);
+    if (timed) (void)hipEventRecord(ev[0], st);
     MeLaunch ml{};
     ml.streams = e->desc;
     ml.mb_stream = e->mb_stream;
@@ -182,9 +193,9 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.fuse_tail = fused;
     ml.xcd_remap = e->xcd_remap;
     if (launch_me(ml, st)) return H261G_ERR_DEVICE;
-    if (timed) (void)hipEventRecord(e->ev[1], st);
+    if (timed) (void)hipEventRecord(ev[1], st);
     if (!fused && launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
-    if (timed) (void)hipEventRecord(e->ev[2], st);
+    if (timed) (void)hipEventRecord(ev[2], st);
     MbLaunch bl{};
     bl.streams = e->desc;
     bl.mb_stream = e->mb_stream;
@@ -197,7 +208,7 @@ int enqueue_step(h261g_engine *e, bool timed)
     bl.mbs_per_wave = e->mbs_per_wave;
     bl.xcd_remap = e->xcd_remap;
     if (launch_mb(bl, st)) return H261G_ERR_DEVICE;
-    if (timed) (void)hipEventRecord(e->ev[3], st);
+    if (timed) (void)hipEventRecord(ev[3], st);
     if (fused) {
         if (launch_frame_tail(e->desc, e->seq, e->dn, e->rc, e->side, ns, st)) return H261G_ERR_DEVICE;
     } else {
@@ -205,22 +216,37 @@ int enqueue_step(h261g_engine *e, bool timed)
         if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st)) return H261G_ERR_DEVICE;
         if (launch_advance(e->desc, e->seq, ns, st)) return H261G_ERR_DEVICE;
     }
-    if (timed) (void)hipEventRecord(e->ev[4], st);
+    if (timed) {
+        (void)hipEventRecord(ev[4], st);
+        e->ev_recorded++;
+    }
     return H261G_OK;
 }
 
-int accumulate_times(h261g_engine *e)
+/* reads back the oldest recorded event set */
+int read_oldest_times(h261g_engine *e)
 {
-    if (!ok(hipEventSynchronize(e->ev[4]), "event")) return H261G_ERR_DEVICE;
+    hipEvent_t *ev = e->ev[e->ev_read % h261g_engine::kEvRing];
+    if (!ok(hipEventSynchronize(ev[4]), "event")) return H261G_ERR_DEVICE;
     float a = 0, b = 0, c = 0, f = 0;
-    (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
-    (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
-    (void)hipEventElapsedTime(&c, e->ev[2], e->ev[3]);
-    (void)hipEventElapsedTime(&f, e->ev[3], e->ev[4]);
+    (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+    (void)hipEventElapsedTime(&f, ev[3], ev[4]);
     e->me_ms += a;
     e->other_ms += b + f;
     e->mb_ms += c;
     e->launches++;
+    e->ev_read++;
+    return H261G_OK;
+}
+
+int read_all_times(h261g_engine *e)
+{
+    while (e->ev_read < e->ev_recorded) {
+        int rcode = read_oldest_times(e);
+        if (rcode) return rcode;
+    }
     return H261G_OK;
 }
 
@@ -302,7 +328,8 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipMalloc((void **)&e->rc, sizeof(RcState) * max_streams), "hipMalloc") &&
                 ok(hipMemset(e->desc, 0, sizeof(StreamDesc) * max_streams), "hipMemset") &&
                 ok(hipMemset(e->seq, 0, sizeof(SeqState) * max_streams), "hipMemset");
-    for (int i = 0; good && i < 5; i++) good = ok(hipEventCreate(&e->ev[i]), "event");
+    for (int r = 0; good && r < h261g_engine::kEvRing; r++)
+        for (int i = 0; good && i < 5; i++) good = ok(hipEventCreate(&e->ev[r][i]), "event");
     if (!good) {
         h261g_engine_destroy(e);
         return H261G_ERR_DEVICE;
@@ -332,8 +359,9 @@ int h261g_engine_destroy(h261g_engine_t *e)
     hipFree(e->side);
     hipFree(e->levels);
     hipHostFree(e->mbstruct);
-    for (int i = 0; i < 5; i++)
-        if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    for (int r = 0; r < h261g_engine::kEvRing; r++)
+        for (int i = 0; i < 5; i++)
+            if (e->ev[r][i]) (void)hipEventDestroy(e->ev[r][i]);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return H261G_OK;
@@ -561,7 +589,7 @@ int h261g_engine_step(h261g_engine_t *e)
         if (s.open && (s.pending || s.seq)) n++;
     int rcode = enqueue_step(e, e->profile);
     if (rcode) return rcode;
-    if (e->profile && (rcode = accumulate_times(e))) return rcode;
+
     for (auto &s : e->slots) s.pending = false;
     return n;
 }
@@ -705,6 +733,8 @@ int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int
 int h261g_engine_profile(h261g_engine_t *e, int32_t enable)
 {
     if (!e) return H261G_ERR_ARG;
+    if (e->ev_read < e->ev_recorded && !ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
+    e->ev_read = e->ev_recorded;
     e->profile = enable != 0;
     e->me_ms = e->mb_ms = e->other_ms = 0;
     e->launches = 0;
@@ -714,6 +744,8 @@ int h261g_engine_profile(h261g_engine_t *e, int32_t enable)
 int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, double *other_ms, int64_t *launches)
 {
     if (!e) return H261G_ERR_ARG;
+    int rcode = read_all_times(e);
+    if (rcode) return rcode;
     if (me_ms) *me_ms = e->me_ms;
     if (mb_ms) *mb_ms = e->mb_ms;
     if (other_ms) *other_ms = e->other_ms;

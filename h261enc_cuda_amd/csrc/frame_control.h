@@ -118,20 +118,40 @@ __device__ __forceinline__ void rc_update_one(RcState &rc, int ft, int bytes)
     rc.vbv_fill -= bytes;
 }
 
-/* all 64 lanes call; returns the frame's total bits on every lane */
+/* all 64 lanes call; returns the frame's total bits on every lane.  The side
+ * information of every GOB is loaded before any is scanned, so the loads
+ * overlap instead of costing one memory latency per GOB. */
 __device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide *side, int picq, int lane)
 {
+    constexpr int kMaxGob = 12;
     const bool cif = d.mb_w == 22;
     const int n_gob = cif ? 12 : 3;
     const int gob_cols = cif ? 2 : 1;
     const uint64_t below = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    const bool valid = lane < 33;
+    /* unconditional loads from clamped in-frame indexes (lanes 33-63 and
+     * GOBs past the last read a valid entry and discard it), so all loads
+     * issue back to back */
+    const int ml = valid ? lane : 32;
+    uint64_t raw[kMaxGob];
+#pragma unroll
+    for (int g = 0; g < kMaxGob; g++) {
+        const int gg = g < n_gob ? g : n_gob - 1;
+        const int gx = (gg % gob_cols) * 11, gy = (gg / gob_cols) * 3;
+        raw[g] = *(const uint64_t *)(side + d.mb_base + (gy + ml / 11) * d.mb_w + gx + ml % 11);
+    }
+    MbSide sds[kMaxGob];
+#pragma unroll
+    for (int g = 0; g < kMaxGob; g++) {
+        const uint64_t v = (valid && g < n_gob) ? raw[g] : 0ull;
+        __builtin_memcpy(&sds[g], &v, sizeof(v));
+    }
     int bits = 0;
-    for (int g = 0; g < n_gob; g++) {
-        const int gx = (g % gob_cols) * 11, gy = (g / gob_cols) * 3;
-        const bool valid = lane < 33;
-        MbSide sd = {};
-        if (valid) sd = side[d.mb_base + (gy + lane / 11) * d.mb_w + gx + lane % 11];
-        const int flags = valid ? sd.flags : 0;
+#pragma unroll
+    for (int g = 0; g < kMaxGob; g++) {
+        if (g >= n_gob) break;
+        const MbSide sd = sds[g];
+        const int flags = sd.flags;
         const bool coded = flags != 0;
         const uint64_t cm = __ballot(coded);
         const uint64_t qm = __ballot(coded && (flags & H261_MB_COEFF));

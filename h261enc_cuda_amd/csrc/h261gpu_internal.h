@@ -79,6 +79,7 @@ struct MbSide {
     int8_t mvx, mvy;
     int16_t bits; /* MTYPE(no MQUANT)+CBP+TCOEFF bits, for the frame-size count */
 };
+static_assert(sizeof(MbSide) == 8, "MbSide is loaded as one 64-bit word");
 
 /* DCT denoise state, device resident; same layout as h261_dct_denoise_t */
 struct DenoiseState {

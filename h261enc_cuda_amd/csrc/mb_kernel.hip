@@ -35,8 +35,15 @@ namespace h261g {
 namespace {
 
 /* float tables initialised from double literals (mbgpu:335, 823) */
-__constant__ float c_fwd_post[8] = {1.0, 0.720960, 0.765367, 0.850430, 1.0, 1.272759, 1.847759, 3.624510};
-__constant__ float c_inv_pre[8] = {1.0, 1.387040, 1.306563, 1.175876, 1.0, 0.785695, 0.541196, 0.275899};
+#define H261G_FWD_POST {1.0, 0.720960, 0.765367, 0.850430, 1.0, 1.272759, 1.847759, 3.624510}
+#define H261G_INV_PRE {1.0, 1.387040, 1.306563, 1.175876, 1.0, 0.785695, 0.541196, 0.275899}
+/* indexed by the lane's column (memory) and by unrolled constants (folded) */
+__constant__ float c_fwd_post[8] = H261G_FWD_POST;
+__constant__ float c_inv_pre[8] = H261G_INV_PRE;
+constexpr float kFwdPost[8] = H261G_FWD_POST;
+constexpr float kInvPre[8] = H261G_INV_PRE;
+#undef H261G_FWD_POST
+#undef H261G_INV_PRE
 
 /* f_inv_quant = 1.0 / (float)(2q) as a double quotient rounded to float
  * (mbgpu:685); folded exactly at compile time */
@@ -118,14 +125,13 @@ __device__ __forceinline__ void idct_core(const float *t, float *o)
     o[7] = tmp0 - tmp13;
 }
 
-__device__ __forceinline__ int tcoeff_bits(int run, int level)
+/* bits of one TCOEFF event: the code and its sign bit, or an escape; tlen
+ * is the wave's LDS copy of the code lengths plus one (kEscapeBits where the
+ * table has no code) */
+__device__ __forceinline__ int tcoeff_bits(const uint8_t *tlen, int run, int level)
 {
-    int a = level < 0 ? -level : level;
-    if (run < kTcoeffRuns && a < kTcoeffLevels) {
-        int len = c_tcoeff[run][a].len;
-        if (len) return len + 1;
-    }
-    return kEscapeBits;
+    const int a = level < 0 ? -level : level;
+    return (run < kTcoeffRuns && a < kTcoeffLevels) ? (int)tlen[run * kTcoeffLevels + a] : kEscapeBits;
 }
 
 /* flushes this wave's denoise statistics of stream s (LDS -> global
@@ -152,6 +158,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     __shared__ int32_t stat[2][64];
     __shared__ int32_t scount[2];
     __shared__ int32_t sblk[8];
+    __shared__ uint8_t tlen[kTcoeffRuns * kTcoeffLevels];
 
     const int lane = threadIdx.x;
     const int b = lane >> 3; /* block 0-5 (6, 7 idle) */
@@ -161,6 +168,15 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     stat[0][lane] = 0;
     stat[1][lane] = 0;
     if (lane < 2) scount[lane] = 0;
+    for (int i = lane; i < kTcoeffRuns * kTcoeffLevels; i += 64) {
+        const int len = c_tcoeff[i / kTcoeffLevels][i % kTcoeffLevels].len;
+        tlen[i] = (uint8_t)(len ? len + 1 : kEscapeBits);
+    }
+    /* per-lane constants of the loop: zig-zag position of lane, column
+     * scale factors of lane (b, c) */
+    const int zz = c_zigzag[lane];
+    const float post_c = (float)((double)c_fwd_post[rc] * 0.125);
+    const float pre_c = c_inv_pre[rc];
 
     const int n_wg = (p.n_mb + p.mbs_per_wave - 1) / p.mbs_per_wave;
     const int wg = launch_index(blockIdx.x, n_wg, p.xcd_remap);
@@ -296,14 +312,13 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             float v[8];
 #pragma unroll
             for (int r = 0; r < 8; r++) v[r] = tf[b * kBlkPitch + r * 8 + c];
-            float cs = c_fwd_post[c];
-            cs = (float)((double)cs * 0.125);
+            const float cs = post_c;
             float o[8];
             fdct_core(v[0] + v[7], v[1] + v[6], v[2] + v[5], v[3] + v[4], v[3] - v[4], v[2] - v[5], v[1] - v[6],
                       v[0] - v[7], o);
 #pragma unroll
             for (int k = 0; k < 8; k++)
-                coef[k] = (int)floorf((float)((double)(cs * c_fwd_post[k] * o[k]) + 0.49999));
+                coef[k] = (int)floorf((float)((double)(cs * kFwdPost[k] * o[k]) + 0.49999));
 
             /* denoise with the previous frame's offsets, staged as UInt16
              * (mbgpu:525-528, 586-604) */
@@ -376,8 +391,8 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             float tv[8], o[8];
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                float cs = (float)((double)c_inv_pre[r] * 0.125);
-                tv[r] = ((float)deq[r] * c_inv_pre[c]) * cs;
+                const float cs = (float)((double)kInvPre[r] * 0.125);
+                tv[r] = ((float)deq[r] * pre_c) * cs;
             }
             idct_core(tv, o);
             __builtin_amdgcn_s_barrier(); /* all phase-2 reads of tf are done */
@@ -412,7 +427,6 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
         const bool written_any = intra || cbp;
         if (p.side && written_any) {
             /* lane k = transmission position k of each block in turn */
-            const int zz = c_zigzag[lane];
             for (int k = 0; k < 6; k++) {
                 const bool blk_written = intra || (cbp & kCbpBit[k]);
                 if (!blk_written) continue;
@@ -428,9 +442,9 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                         else if (l == 1 || l == -1)
                             bits += 2; /* first coefficient '1s' */
                         else
-                            bits += tcoeff_bits(0, l);
+                            bits += tcoeff_bits(tlen, 0, l);
                     } else {
-                        bits += tcoeff_bits(run, l);
+                        bits += tcoeff_bits(tlen, run, l);
                     }
                 }
                 if (lane == 0) bits += 2; /* EOB */

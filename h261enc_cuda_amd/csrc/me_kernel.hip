@@ -32,21 +32,22 @@ namespace {
 constexpr int kWinRows = 46;  /* 16 + 2*15 */
 constexpr int kWinDw = 12;    /* 48 bytes per row: frame cols bx-16 .. bx+31 */
 
+/* minimum over the wave: quads, half rows and rows by DPP, rows by readlane */
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ int group16_sum(int v)
-{
-#pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    uint32_t o;
+    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, false); /* quad_perm [1,0,3,2] */
+    v = o < v ? o : v;
+    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, false); /* quad_perm [2,3,0,1] */
+    v = o < v ? o : v;
+    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false); /* row_half_mirror */
+    v = o < v ? o : v;
+    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false); /* row_mirror */
+    v = o < v ? o : v;
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    const uint32_t a = r0 < r1 ? r0 : r1, b = r2 < r3 ? r2 : r3;
+    return a < b ? a : b;
 }
 
 /* Z-order index with x in the even bits: the reference reduction order */
@@ -75,50 +76,98 @@ __device__ __forceinline__ int coarse_vec(int i)
     return i < 5 ? 3 * i - 14 : (i == 5 ? 0 : 3 * i - 16);
 }
 
-/* 16x16 SAD of the candidate (dx, dy) against the staged window */
-__device__ __forceinline__ uint32_t sad16(const uint32_t *win, const uint32_t *cur, int dx, int dy)
+/* one row of a candidate: window dwords o..o+4 realigned by sh bytes against
+ * the current row's four dwords (scalar operands) */
+__device__ __forceinline__ uint32_t sad_row(const uint32_t *row, int sh, uint32_t c0, uint32_t c1, uint32_t c2,
+                                            uint32_t c3, uint32_t acc)
 {
-    int c0 = dx + 16;
-    int q = c0 >> 2, sh = c0 & 3;
-    const uint32_t *row = win + (dy + 15) * kWinDw + q;
-    uint32_t acc = 0;
-#pragma unroll 4
-    for (int k = 0; k < 16; k++) {
-        uint32_t w0 = row[0], w1 = row[1], w2 = row[2], w3 = row[3], w4 = row[4];
-        uint32_t r0 = __builtin_amdgcn_alignbyte(w1, w0, sh);
-        uint32_t r1 = __builtin_amdgcn_alignbyte(w2, w1, sh);
-        uint32_t r2 = __builtin_amdgcn_alignbyte(w3, w2, sh);
-        uint32_t r3 = __builtin_amdgcn_alignbyte(w4, w3, sh);
-        acc = __builtin_amdgcn_sad_u8(r0, cur[4 * k + 0], acc);
-        acc = __builtin_amdgcn_sad_u8(r1, cur[4 * k + 1], acc);
-        acc = __builtin_amdgcn_sad_u8(r2, cur[4 * k + 2], acc);
-        acc = __builtin_amdgcn_sad_u8(r3, cur[4 * k + 3], acc);
-        row += kWinDw;
-    }
+    const uint32_t w0 = row[0], w1 = row[1], w2 = row[2], w3 = row[3], w4 = row[4];
+    acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(w1, w0, sh), c0, acc);
+    acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(w2, w1, sh), c1, acc);
+    acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(w3, w2, sh), c2, acc);
+    acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(w4, w3, sh), c3, acc);
     return acc;
 }
 
-/* megpu_satd_4x4_kernel (src/megpu_kernels.cuh:467-500) */
-__device__ __forceinline__ int satd4x4(const int d[16])
+/* 16x16 SADs of NS candidates per lane against the staged window.  The
+ * current MB's rows are wave-uniform: they are read with scalar loads (the
+ * constant address space) and enter v_sad_u8 as scalar operands. */
+typedef const __attribute__((address_space(4))) uint32_t *scalar_u32_ptr;
+
+template <int NS>
+__device__ __forceinline__ void sad16(const uint32_t *win, const uint8_t *cur_mb, int pitch, const int (&off)[NS],
+                                      const int (&sh)[NS], uint32_t (&acc)[NS])
 {
-    int t[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int a1 = d[4 * i] + d[4 * i + 1], a2 = d[4 * i] - d[4 * i + 1];
-        int a3 = d[4 * i + 2] + d[4 * i + 3], a4 = d[4 * i + 2] - d[4 * i + 3];
-        t[i][0] = a1 + a3;
-        t[i][1] = a2 + a4;
-        t[i][2] = a1 - a3;
-        t[i][3] = a2 - a4;
-    }
-    int s = 0;
+    for (int i = 0; i < NS; i++) acc[i] = 0;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int a1 = t[0][i] + t[1][i], a2 = t[0][i] - t[1][i];
-        int a3 = t[2][i] + t[3][i], a4 = t[2][i] - t[3][i];
-        s += abs(a1 + a3) + abs(a2 + a4) + abs(a1 - a3) + abs(a2 - a4);
+    for (int k = 0; k < 16; k++) {
+        const scalar_u32_ptr c = (scalar_u32_ptr)(cur_mb + (size_t)k * pitch);
+        const uint32_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+#pragma unroll
+        for (int i = 0; i < NS; i++) acc[i] = sad_row(win + off[i] + k * kWinDw, sh[i], c0, c1, c2, c3, acc[i]);
     }
-    return s;
+}
+
+/* window offset and byte shift of candidate (vx, vy); candidates outside
+ * the limits are evaluated at (0, 0) (always inside the window) and their
+ * cost replaced by 65535 */
+__device__ __forceinline__ void cand_addr(int vx, int vy, bool valid, int &off, int &sh)
+{
+    const int x = valid ? vx : 0, y = valid ? vy : 0;
+    off = (y + 15) * kWinDw + ((x + 16) >> 2);
+    sh = (x + 16) & 3;
+}
+
+typedef short short2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ short2v as_s2(uint32_t v)
+{
+    short2v r;
+    __builtin_memcpy(&r, &v, 4);
+    return r;
+}
+
+/* partner value across the lanes of a quad (DPP quad_perm) */
+template <int kCtrl>
+__device__ __forceinline__ short2v quad_swap(short2v v)
+{
+    int x;
+    __builtin_memcpy(&x, &v, 4);
+    x = __builtin_amdgcn_mov_dpp(x, kCtrl, 0xf, 0xf, false);
+    short2v r;
+    __builtin_memcpy(&r, &x, 4);
+    return r;
+}
+
+/* 4x4 Hadamard of a sub-block held by the 4 lanes of a quad, lane y = row y,
+ * as pairs (d0, d2), (d1, d3) of packed int16; returns this lane's sum of
+ * |coefficient| over its 4 coefficients (the sub-block's SATD summed over
+ * the quad is megpu_satd_4x4_kernel, src/megpu_kernels.cuh:467-500) */
+__device__ __forceinline__ int satd_quad(short2v e, short2v o, short2v s1, short2v s2)
+{
+    const short2v sum = e + o, dif = e - o;   /* (a1, a3), (a2, a4) */
+    const short2v u = {sum.x, dif.x}, v = {sum.y, dif.y};
+    short2v p = u + v, m = u - v;             /* (t0, t1), (t2, t3) */
+    /* columns: butterflies across the quad; an odd partner negates itself */
+    p = p * s1 + quad_swap<0xB1>(p);
+    m = m * s1 + quad_swap<0xB1>(m);
+    p = p * s2 + quad_swap<0x4E>(p);
+    m = m * s2 + quad_swap<0x4E>(m);
+    const short2v ap = __builtin_elementwise_max(p, -p), am = __builtin_elementwise_max(m, -m);
+    const short2v t = ap + am;
+    return (int)(uint16_t)t.x + (int)(uint16_t)t.y;
+}
+
+/* sum over the wave: quads, half rows and rows by DPP, rows by readlane */
+__device__ __forceinline__ int wave_sum(int x)
+{
+    x += __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false);  /* quad_perm [1,0,3,2] */
+    x += __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false);  /* quad_perm [2,3,0,1] */
+    x += __builtin_amdgcn_mov_dpp(x, 0x141, 0xf, 0xf, false); /* row_half_mirror */
+    x += __builtin_amdgcn_mov_dpp(x, 0x140, 0xf, 0xf, false); /* row_mirror */
+    return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) + __builtin_amdgcn_readlane(x, 32) +
+           __builtin_amdgcn_readlane(x, 48);
 }
 
 /* the full SAD surface of one MB for dy, dx in [-15, 15] (dx = -16 is a
@@ -156,7 +205,7 @@ template <int kVariant>
 __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
 {
     __shared__ __align__(16) uint32_t win[kWinRows * kWinDw];
-    __shared__ __align__(16) uint32_t curw[64];
+    __shared__ __align__(16) uint32_t curw[kVariant == kMeSurface ? 64 : 4];
     __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
 
     const int g = launch_index(blockIdx.x, p.n_mb, p.xcd_remap);
@@ -171,6 +220,7 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
     const int bx = (m % d.mb_w) * 16, by = (m / d.mb_w) * 16;
     const uint8_t *cur = d.cur[0];
     const uint8_t *ref = d.ref[0];
+    const uint8_t *cur_mb = cur + (size_t)by * W + bx;
 
     /* motion vector limits (gpu_device.cu:226-265) */
     const int lx0 = max(bx - 15, 0) - bx, lx1 = min(bx + 15, (d.mb_w - 1) * 16) - bx;
@@ -196,7 +246,7 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
         const int i = lane + 64 * j;
         if (i < kWinWords) win[i] = staged[j];
     }
-    curw[lane] = cw;
+    if (kVariant == kMeSurface) curw[lane] = cw;
     __syncthreads();
 
     const bool search = !(p.skip_search_on_intra && d.frame_type == H261_FRAME_TYPE_INTRA);
@@ -206,18 +256,41 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
             sad_surface(win, curw, surf, lane);
             __syncthreads();
         }
+        /* surface variant: the cost of an in-limit candidate is a lookup */
         auto cost_of = [&](int vx, int vy) -> uint32_t {
             if (!(vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1)) return 65535u;
-            return kVariant == kMeSurface ? (uint32_t)surf[(vy + 15) * 32 + vx + 16] : sad16(win, curw, vx, vy);
+            return (uint32_t)surf[(vy + 15) * 32 + vx + 16];
         };
-        /* stage 1: coarse grid */
+        /* stage 1: coarse grid, candidates lane and lane + 64 */
         uint32_t best = 0xffffffffu;
-        for (int ci = lane; ci < 121; ci += 64) {
-            int tx = ci % 11, ty = ci / 11;
-            int vx = coarse_vec(tx), vy = coarse_vec(ty);
-            uint32_t cost = cost_of(vx, vy);
-            uint32_t key = (cost << 8) | morton((uint32_t)tx, (uint32_t)ty);
-            best = key < best ? key : best;
+        if (kVariant == kMeSurface) {
+            for (int ci = lane; ci < 121; ci += 64) {
+                const int tx = ci % 11, ty = ci / 11;
+                const uint32_t cost = cost_of(coarse_vec(tx), coarse_vec(ty));
+                const uint32_t key = (cost << 8) | morton((uint32_t)tx, (uint32_t)ty);
+                best = key < best ? key : best;
+            }
+        } else {
+            int tx[2], ty[2], off[2], sh[2];
+            bool valid[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int ci = min(lane + 64 * i, 120);
+                tx[i] = ci % 11;
+                ty[i] = ci / 11;
+                const int vx = coarse_vec(tx[i]), vy = coarse_vec(ty[i]);
+                valid[i] = vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1 && lane + 64 * i < 121;
+                cand_addr(vx, vy, valid[i], off[i], sh[i]);
+            }
+            uint32_t acc[2];
+            sad16<2>(win, cur_mb, W, off, sh, acc);
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const uint32_t cost = valid[i] ? acc[i] : 65535u;
+                const uint32_t key = (cost << 8) | morton((uint32_t)tx[i], (uint32_t)ty[i]);
+                /* lanes 57-63 hold a duplicate of candidate 120 in slot 1 */
+                if (lane + 64 * i < 121) best = key < best ? key : best;
+            }
         }
         best = wave_min_u32(best);
         int wtx, wty;
@@ -225,12 +298,21 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
         const int sx = coarse_vec(wtx), sy = coarse_vec(wty);
 
         /* stage 2: 7x7 refine around the coarse winner */
-        best = 0xffffffffu;
-        if (lane < 49) {
-            int tx = lane % 7, ty = lane / 7;
-            int vx = sx + tx - 3, vy = sy + ty - 3;
-            uint32_t cost = cost_of(vx, vy);
-            best = (cost << 8) | morton((uint32_t)tx, (uint32_t)ty);
+        {
+            const int tx = (lane < 49 ? lane : 0) % 7, ty = (lane < 49 ? lane : 0) / 7;
+            const int vx = sx + tx - 3, vy = sy + ty - 3;
+            uint32_t cost;
+            if (kVariant == kMeSurface) {
+                cost = cost_of(vx, vy);
+            } else {
+                const bool valid = vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1;
+                int off[1], sh[1];
+                cand_addr(vx, vy, valid, off[0], sh[0]);
+                uint32_t acc[1];
+                sad16<1>(win, cur_mb, W, off, sh, acc);
+                cost = valid ? acc[0] : 65535u;
+            }
+            best = lane < 49 ? (cost << 8) | morton((uint32_t)tx, (uint32_t)ty) : 0xffffffffu;
         }
         best = wave_min_u32(best);
         unmorton(best & 0xffu, wtx, wty);
@@ -238,41 +320,54 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
         mvy = sy + wty - 3;
     }
 
-    /* SATD costs: lanes 0-15 inter, 16-31 inter+filter, 32-47 intra */
-    int satd = 0;
-    const int kind = lane >> 4;
-    if (lane < 48 && (search || kind == 2)) {
-        const int sub = lane & 15, sub_x = sub & 3, sub_y = sub >> 2;
-        const uint8_t *cb = (const uint8_t *)curw;
-        const uint8_t *wb = (const uint8_t *)win;
-        int dd[16];
-        /* window byte of MB pel (x, y) displaced by the mv */
-        const int wr0 = mvy + 15 + 4 * sub_y, wc0 = mvx + 16 + 4 * sub_x;
+    /* SATD costs (inter, inter + loop filter, intra): lane = (4x4 sub-block,
+     * row y); 4 pels per lane as packed int16 pairs (p0, p2), (p1, p3) */
+    const int sub = lane >> 2, yq = lane & 3, sub_x = sub & 3, sub_y = sub >> 2;
+    const int R = 4 * sub_y + yq; /* MB row */
+    const uint32_t crow = (uint32_t)__shfl((int)cw, 4 * R + sub_x, 64);
+    const short2v ce = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c020c00u));
+    const short2v co = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c030c01u));
+    /* window: row of the predicted MB row, byte of pel -1 of this lane */
+    const int wr = mvy + 15 + R;
+    const int b0 = mvx + 16 + 4 * sub_x - 1;
+    const int q = b0 >> 2, bsh = b0 & 3, q2 = min(q + 2, kWinDw - 1);
+    const int yl = R & 7;
+    const bool row_border = yl == 0 || yl == 7;
+    /* [1 2 1] along the row, as packed pairs (h0, h2) and (h1, h3) */
+    uint32_t h02[3], h13[3], pm02 = 0, pm13 = 0;
 #pragma unroll
-        for (int y = 0; y < 4; y++)
-#pragma unroll
-            for (int x = 0; x < 4; x++) {
-                int c = cb[(4 * sub_y + y) * 16 + 4 * sub_x + x];
-                dd[4 * y + x] = kind == 2 ? c : c - (int)wb[(wr0 + y) * 48 + wc0 + x];
-            }
-        if (kind == 1) {
-            /* 3x3 [1 2 1] filter on the 6x6 interior of each 8x8 block
-             * (megpu:833-879); border pels stay unfiltered */
-            const int fx0 = (sub_x & 1) ? 0 : 1, fx1 = (sub_x & 1) ? 3 : 4;
-            const int fy0 = (sub_y & 1) ? 0 : 1, fy1 = (sub_y & 1) ? 3 : 4;
-            for (int y = fy0; y < fy1; y++)
-                for (int x = fx0; x < fx1; x++) {
-                    const uint8_t *r = wb + (wr0 + y) * 48 + wc0 + x;
-                    int f = r[-49] + 2 * r[-48] + r[-47] + 2 * r[-1] + 4 * r[0] + 2 * r[1] + r[47] +
-                            2 * r[48] + r[49];
-                    f = (f + 8) >> 4;
-                    dd[4 * y + x] = (int)cb[(4 * sub_y + y) * 16 + 4 * sub_x + x] - f;
-                }
+    for (int i = 0; i < 3; i++) {
+        const int r = row_border ? wr : wr - 1 + i;
+        const uint32_t *w = win + r * kWinDw;
+        const uint32_t w0 = w[q], w1 = w[q + 1], w2 = w[q2];
+        const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, bsh); /* pels -1, 0, 1, 2 */
+        const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, bsh); /* pels 3, 4 */
+        const uint32_t A = __builtin_amdgcn_perm(hi, lo, 0x0c020c00u); /* (p-1, p1) */
+        const uint32_t B = __builtin_amdgcn_perm(hi, lo, 0x0c030c01u); /* (p0, p2) */
+        const uint32_t C = __builtin_amdgcn_perm(hi, lo, 0x0c040c02u); /* (p1, p3) */
+        const uint32_t D = __builtin_amdgcn_perm(hi, lo, 0x0c050c03u); /* (p2, p4) */
+        h02[i] = A + 2 * B + C;
+        h13[i] = B + 2 * C + D;
+        if (i == 1) {
+            pm02 = B;
+            pm13 = C;
         }
-        satd = satd4x4(dd);
     }
-    satd = group16_sum(satd);
-    const int c_inter = __shfl(satd, 0, 64), c_filter = __shfl(satd, 16, 64), c_intra = __shfl(satd, 32, 64);
+    /* 3x3 [1 2 1] filter, rounded, on the 6x6 interior of each 8x8 block
+     * (megpu:833-879); pels on the block border keep the raw prediction */
+    const uint32_t f02 = (((h02[0] + 2 * h02[1] + h02[2]) + 0x00080008u) >> 4) & 0x00ff00ffu;
+    const uint32_t f13 = (((h13[0] + 2 * h13[1] + h13[2]) + 0x00080008u) >> 4) & 0x00ff00ffu;
+    const uint32_t m02 = row_border ? 0xffffffffu : ((sub_x & 1) ? 0u : 0x0000ffffu);
+    const uint32_t m13 = row_border ? 0xffffffffu : ((sub_x & 1) ? 0xffff0000u : 0u);
+    const uint32_t fp02 = (pm02 & m02) | (f02 & ~m02), fp13 = (pm13 & m13) | (f13 & ~m13);
+    const short2v s1 = (yq & 1) ? short2v{-1, -1} : short2v{1, 1};
+    const short2v s2 = (yq & 2) ? short2v{-1, -1} : short2v{1, 1};
+    const int t_inter = satd_quad(ce - as_s2(pm02), co - as_s2(pm13), s1, s2);
+    const int t_filter = satd_quad(ce - as_s2(fp02), co - as_s2(fp13), s1, s2);
+    const int t_intra = satd_quad(ce, co, s1, s2);
+    const int c_inter = search ? wave_sum(t_inter) : 0;
+    const int c_filter = search ? wave_sum(t_filter) : 0;
+    const int c_intra = wave_sum(t_intra);
 
     if (lane == 0) {
         MeResult r;
