@@ -37,11 +37,13 @@ METRIC = "CIF frames/s on 1 MI355X, batched vs plain, bit-exact; p99 frame laten
 HBM_PEAK_GBS = 8000.0
 SAD_PEAK = 157.3e12  # byte-SADs/s: 256 CU x 64 lanes x 2.4 GHz x 4 (v_sad_u8)
 
-# algorithmic bytes per frame (cur + ref luma + 16 B/MB results for motion
-# estimation; cur + ref YUV read, recon YUV written, int16 levels + 16 B/MB
-# side information for the macroblock coder)
-ME_BYTES = {396: 352 * 288 * 2 + 16 * 396, 99: 176 * 144 * 2 + 16 * 99}
-MB_BYTES = {396: 3 * 152064 + 396 * (768 + 16), 99: 3 * 38016 + 99 * (768 + 16)}
+# algorithmic bytes per frame.  Motion estimation: read the current and the
+# reference luma, write 44 B/MB (the ME result, 20 B, and the coding mode,
+# 24 B).  Macroblock coder: read the current and reference YUV and the 24 B
+# mode, write the reconstructed YUV, the int8 levels (384 B) and the 8 B side
+# information per MB.
+ME_BYTES = {396: 352 * 288 * 2 + 44 * 396, 99: 176 * 144 * 2 + 44 * 99}
+MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 384 + 8)}
 
 
 # HBM bytes per macroblock actually moved by each kernel (FETCH_SIZE +
@@ -158,8 +160,8 @@ def algorithmic(specs, t0, t1):
         n = max(0, min(s["length"], t1) - t0)
         intra = 1 if (t0 == 0 and n > 0) else 0
         inter = n - intra
-        me += inter * ME_BYTES[nmb] + intra * (w * h + 16 * nmb)
-        mb += inter * MB_BYTES[nmb] + intra * (2 * fsz + nmb * (768 + 16))
+        me += inter * ME_BYTES[nmb] + intra * (w * h + 44 * nmb)
+        mb += inter * MB_BYTES[nmb] + intra * (2 * fsz + nmb * (24 + 384 + 8))
         sad += inter * SAD_OPS[nmb]
     return me, mb, sad
 

@@ -70,10 +70,15 @@ using namespace h261g;
 struct h261g_engine {
     int max_streams = 0;
     int64_t budget = 0, used = 0;
-    int mbs_per_wave = 1;
+    int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_variant = kMeTwoStage;
     int fused_tails = 1;
-    int xcd_remap = 0;
+    /* bit 0: motion estimation, bit 1: macroblock coder.  Measured at 512
+     * streams: the remap cuts the coder's L2 misses ~9x and its time ~4% (at
+     * 4 MBs per wave); it cuts motion estimation's misses ~7.6x but not its
+     * time (that kernel is VALU/LDS bound, its re-reads are absorbed below
+     * L2), so only the coder is remapped by default */
+    int xcd_remap = 2;
     hipStream_t stream = nullptr;
     std::vector<Slot> slots;
     /* device arrays indexed by stream slot */
@@ -87,7 +92,7 @@ struct h261g_engine {
     MeResult *me = nullptr;
     MbMode *mode = nullptr;
     MbSide *side = nullptr;
-    int16_t *levels = nullptr;
+    int8_t *levels = nullptr; /* [mb][6][64]; intra DC levels (1..255) as uint8 */
     std::vector<int> free_cif, free_qcif; /* MB ranges of closed streams */
     h261_macroblock_t *mbstruct = nullptr; /* scratch for read_macroblocks */
     int mbstruct_cap = 0;
@@ -129,12 +134,12 @@ int ensure_mb_capacity(h261g_engine *e, int n)
     MeResult *me = nullptr;
     MbMode *mo = nullptr;
     MbSide *sd = nullptr;
-    int16_t *lv = nullptr;
+    int8_t *lv = nullptr;
     if (!ok(hipMalloc((void **)&ms, sizeof(int32_t) * cap), "hipMalloc") ||
         !ok(hipMalloc((void **)&me, sizeof(MeResult) * cap), "hipMalloc") ||
         !ok(hipMalloc((void **)&mo, sizeof(MbMode) * cap), "hipMalloc") ||
         !ok(hipMalloc((void **)&sd, sizeof(MbSide) * cap), "hipMalloc") ||
-        !ok(hipMalloc((void **)&lv, sizeof(int16_t) * 384 * (size_t)cap), "hipMalloc")) {
+        !ok(hipMalloc((void **)&lv, (size_t)384 * cap), "hipMalloc")) {
         hipFree(ms);
         hipFree(me);
         hipFree(mo);
@@ -168,6 +173,10 @@ int stream_count(const h261g_engine *e) { return e->max_streams; }
 
 int read_oldest_times(h261g_engine *e);
 
+/* waves of the macroblock coder that keep every SIMD of the device busy
+ * (256 CUs x 4 SIMDs x 5 waves) */
+constexpr int kFillWaves = 256 * 4 * 5;
+
 /* one frame step: three launches (default), or the six standalone launches */
 int enqueue_step(h261g_engine *e, bool timed)
 {
@@ -191,7 +200,7 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.skip_search_on_intra = 1;
     ml.variant = e->me_variant;
     ml.fuse_tail = fused;
-    ml.xcd_remap = e->xcd_remap;
+    ml.xcd_remap = e->xcd_remap & 1;
     if (launch_me(ml, st)) return H261G_ERR_DEVICE;
     if (timed) (void)hipEventRecord(ev[1], st);
     if (!fused && launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
@@ -205,8 +214,11 @@ int enqueue_step(h261g_engine *e, bool timed)
     bl.rc = e->rc;
     bl.side = e->side;
     bl.levels = e->levels;
-    bl.mbs_per_wave = e->mbs_per_wave;
-    bl.xcd_remap = e->xcd_remap;
+    /* 0 = by batch size: one MB per wave while the launch is too small to
+     * fill the device, up to four when there are MBs to spare (fewer waves,
+     * fewer statistic flushes) */
+    bl.mbs_per_wave = e->mbs_per_wave ? e->mbs_per_wave : (n_mb >= 4 * kFillWaves ? 4 : (n_mb >= 2 * kFillWaves ? 2 : 1));
+    bl.xcd_remap = (e->xcd_remap >> 1) & 1;
     if (launch_mb(bl, st)) return H261G_ERR_DEVICE;
     if (timed) (void)hipEventRecord(ev[3], st);
     if (fused) {
@@ -269,7 +281,7 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         e->me_variant = value;
         break;
     case H261G_OPT_MBS_PER_WAVE:
-        if (value < 1 || value > 64) return H261G_ERR_ARG;
+        if (value < 0 || value > 64) return H261G_ERR_ARG;
         e->mbs_per_wave = value;
         break;
     case H261G_OPT_FUSED_TAILS:
@@ -277,7 +289,7 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         e->fused_tails = value;
         break;
     case H261G_OPT_XCD_REMAP:
-        if (value != 0 && value != 1) return H261G_ERR_ARG;
+        if (value < 0 || value > 3) return H261G_ERR_ARG;
         e->xcd_remap = value;
         break;
     default:
@@ -319,7 +331,7 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
     h261g_engine *e = new h261g_engine();
     e->max_streams = max_streams;
     e->budget = hbm_budget;
-    e->mbs_per_wave = mbs_per_wave ? mbs_per_wave : 2;
+    e->mbs_per_wave = mbs_per_wave;
     e->slots.resize(max_streams);
     bool good = ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "stream") &&
                 ok(hipMalloc((void **)&e->desc, sizeof(StreamDesc) * max_streams), "hipMalloc") &&
@@ -651,9 +663,9 @@ int h261g_stream_read_macroblocks(h261g_engine_t *e, int32_t id, h261_macroblock
     Slot *s = get_slot(e, id);
     if (!s || !mbs) return H261G_ERR_ARG;
     std::vector<MbSide> sd(s->n_mb);
-    std::vector<int16_t> lv((size_t)s->n_mb * 384);
+    std::vector<int8_t> lv((size_t)s->n_mb * 384);
     if (!ok(hipMemcpy(sd.data(), e->side + s->mb_base, sizeof(MbSide) * s->n_mb, hipMemcpyDeviceToHost), "hipMemcpy") ||
-        !ok(hipMemcpy(lv.data(), e->levels + (size_t)s->mb_base * 384, sizeof(int16_t) * 384 * s->n_mb,
+        !ok(hipMemcpy(lv.data(), e->levels + (size_t)s->mb_base * 384, (size_t)384 * s->n_mb,
                       hipMemcpyDeviceToHost),
             "hipMemcpy"))
         return H261G_ERR_DEVICE;
@@ -667,7 +679,10 @@ int h261g_stream_read_macroblocks(h261g_engine_t *e, int32_t id, h261_macroblock
         o.rgi_mv[0] = q.mvx;
         o.rgi_mv[1] = q.mvy;
         o.i_coded_block_pattern = q.cbp;
-        for (int k = 0; k < 384; k++) o.rgi_tcoeff[k / 64][k % 64] = lv[(size_t)m * 384 + k];
+        const int8_t *l = lv.data() + (size_t)m * 384;
+        const bool intra = q.type == H261_MB_TYPE_INTRA;
+        for (int k = 0; k < 384; k++)
+            o.rgi_tcoeff[k / 64][k % 64] = (intra && k % 64 == 0) ? (int)(uint8_t)l[k] : (int)l[k];
         o.i_mb_x = (m % s->mb_w) * 16;
         o.i_mb_y = (m / s->mb_w) * 16;
     }

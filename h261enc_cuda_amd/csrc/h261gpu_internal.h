@@ -161,7 +161,7 @@ struct MbLaunch {
     DenoiseState *denoise;        /* per stream */
     RcState *rc;                  /* per stream: kStreamDeviceQuant, fused tail */
     MbSide *side;                 /* optional [n_mb] */
-    int16_t *levels;              /* optional [n_mb][6][64] */
+    int8_t *levels;               /* optional [n_mb][6][64]; intra DC (1..255) as uint8 */
     h261_macroblock_t *mb_struct; /* optional [n_mb] legacy layout */
     int32_t mbs_per_wave;         /* MBs coded by one wave in sequence */
     int32_t xcd_remap;            /* 1: contiguous macroblock runs per XCD (xcd_logical) */

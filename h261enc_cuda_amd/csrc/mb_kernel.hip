@@ -58,7 +58,25 @@ __constant__ float c_inv_quant[32] = {
 #undef H261G_INVQ
 
 __constant__ uint8_t c_zigzag[64] = H261G_ZIGZAG;
-__constant__ Vlc c_tcoeff[kTcoeffRuns][kTcoeffLevels] = H261G_TCOEFF_TABLE;
+
+/* bits of each TCOEFF (run, |level|) event: code length plus the sign bit,
+ * kEscapeBits where the table has no code; four entries per dword */
+constexpr Vlc kTcoeff[kTcoeffRuns][kTcoeffLevels] = H261G_TCOEFF_TABLE;
+constexpr int kTlenWords = kTcoeffRuns * kTcoeffLevels / 4; /* 108 */
+struct TlenTable {
+    uint32_t w[kTlenWords];
+};
+constexpr TlenTable make_tlen()
+{
+    TlenTable t{};
+    for (int i = 0; i < kTcoeffRuns * kTcoeffLevels; i++) {
+        const int len = kTcoeff[i / kTcoeffLevels][i % kTcoeffLevels].len;
+        const uint32_t b = (uint32_t)(len ? len + 1 : kEscapeBits);
+        t.w[i / 4] |= b << (8 * (i % 4));
+    }
+    return t;
+}
+__constant__ TlenTable c_tlen = make_tlen();
 __constant__ Vlc c_cbp[64] = H261G_CBP_TABLE;
 
 constexpr int kCbpBit[6] = {32, 16, 8, 4, 2, 1};
@@ -154,11 +172,12 @@ __device__ __forceinline__ void flush_segment(const MbLaunch &p, int s, int32_t 
 __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 {
     __shared__ float tf[6 * kBlkPitch];
-    __shared__ int32_t lv[6 * 64];
+    __shared__ __align__(16) int32_t lv[6 * 64];
     __shared__ int32_t stat[2][64];
     __shared__ int32_t scount[2];
     __shared__ int32_t sblk[8];
-    __shared__ uint8_t tlen[kTcoeffRuns * kTcoeffLevels];
+    __shared__ __align__(16) uint32_t tlen32[kTlenWords];
+    const uint8_t *tlen = (const uint8_t *)tlen32;
 
     const int lane = threadIdx.x;
     const int b = lane >> 3; /* block 0-5 (6, 7 idle) */
@@ -168,10 +187,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     stat[0][lane] = 0;
     stat[1][lane] = 0;
     if (lane < 2) scount[lane] = 0;
-    for (int i = lane; i < kTcoeffRuns * kTcoeffLevels; i += 64) {
-        const int len = c_tcoeff[i / kTcoeffLevels][i % kTcoeffLevels].len;
-        tlen[i] = (uint8_t)(len ? len + 1 : kEscapeBits);
-    }
+    if (lane < kTlenWords / 2) ((uint2 *)tlen32)[lane] = ((const uint2 *)c_tlen.w)[lane];
     /* per-lane constants of the loop: zig-zag position of lane, column
      * scale factors of lane (b, c) */
     const int zz = c_zigzag[lane];
@@ -458,9 +474,18 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
         /* ---- outputs ---- */
         const int flags = md.flags | (cbp ? H261_MB_COEFF : 0);
         if (p.levels) {
-            int16_t *out = p.levels + (size_t)g * 384;
+            /* 384 levels as 96 dwords of 4 bytes: AC levels are within
+             * [-127, 127], an intra DC within [1, 255] (read back as uint8) */
+            uint32_t *out = (uint32_t *)(p.levels + (size_t)g * 384);
 #pragma unroll
-            for (int k = 0; k < 6; k++) out[k * 64 + lane] = (int16_t)lv[k * 64 + lane];
+            for (int j = 0; j < 2; j++) {
+                const int w = lane + 64 * j;
+                if (w < 96) {
+                    const int4 v = *(const int4 *)&lv[4 * w];
+                    out[w] = (uint32_t)(v.x & 0xff) | ((uint32_t)(v.y & 0xff) << 8) | ((uint32_t)(v.z & 0xff) << 16) |
+                             ((uint32_t)(v.w & 0xff) << 24);
+                }
+            }
         }
         if (p.mb_struct) {
             h261_macroblock_t *o = p.mb_struct + g;

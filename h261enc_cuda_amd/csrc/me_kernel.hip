@@ -31,6 +31,15 @@ namespace {
 
 constexpr int kWinRows = 46;  /* 16 + 2*15 */
 constexpr int kWinDw = 12;    /* 48 bytes per row: frame cols bx-16 .. bx+31 */
+/* LDS dwords per window row: 15 for the two-stage search (with the lanes of
+ * a coarse or refine pass spread over rows 2-3 apart, a 15-dword pitch puts
+ * their reads in distinct banks more often than 12 does), 12 for the surface
+ * variant, whose 16-byte row loads need aligned rows */
+template <int kVariant>
+constexpr int win_pitch()
+{
+    return kVariant == 1 ? kWinDw : 15;
+}
 
 /* minimum over the wave: quads, half rows and rows by DPP, rows by readlane */
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
@@ -94,7 +103,7 @@ __device__ __forceinline__ uint32_t sad_row(const uint32_t *row, int sh, uint32_
  * constant address space) and enter v_sad_u8 as scalar operands. */
 typedef const __attribute__((address_space(4))) uint32_t *scalar_u32_ptr;
 
-template <int NS>
+template <int NS, int kPitch>
 __device__ __forceinline__ void sad16(const uint32_t *win, const uint8_t *cur_mb, int pitch, const int (&off)[NS],
                                       const int (&sh)[NS], uint32_t (&acc)[NS])
 {
@@ -105,17 +114,18 @@ __device__ __forceinline__ void sad16(const uint32_t *win, const uint8_t *cur_mb
         const scalar_u32_ptr c = (scalar_u32_ptr)(cur_mb + (size_t)k * pitch);
         const uint32_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
 #pragma unroll
-        for (int i = 0; i < NS; i++) acc[i] = sad_row(win + off[i] + k * kWinDw, sh[i], c0, c1, c2, c3, acc[i]);
+        for (int i = 0; i < NS; i++) acc[i] = sad_row(win + off[i] + k * kPitch, sh[i], c0, c1, c2, c3, acc[i]);
     }
 }
 
 /* window offset and byte shift of candidate (vx, vy); candidates outside
  * the limits are evaluated at (0, 0) (always inside the window) and their
  * cost replaced by 65535 */
+template <int kPitch>
 __device__ __forceinline__ void cand_addr(int vx, int vy, bool valid, int &off, int &sh)
 {
     const int x = valid ? vx : 0, y = valid ? vy : 0;
-    off = (y + 15) * kWinDw + ((x + 16) >> 2);
+    off = (y + 15) * kPitch + ((x + 16) >> 2);
     sh = (x + 16) & 3;
 }
 
@@ -204,7 +214,8 @@ __device__ __forceinline__ void sad_surface(const uint32_t *win, const uint32_t 
 template <int kVariant>
 __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
 {
-    __shared__ __align__(16) uint32_t win[kWinRows * kWinDw];
+    constexpr int kPitch = win_pitch<kVariant>();
+    __shared__ __align__(16) uint32_t win[kWinRows * kPitch];
     __shared__ __align__(16) uint32_t curw[kVariant == kMeSurface ? 64 : 4];
     __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
 
@@ -226,25 +237,25 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
     const int lx0 = max(bx - 15, 0) - bx, lx1 = min(bx + 15, (d.mb_w - 1) * 16) - bx;
     const int ly0 = max(by - 15, 0) - by, ly1 = min(by + 15, (d.mb_h - 1) * 16) - by;
 
-    /* stage the window: all loads issued before any wait.  Window dwords
-     * outside the frame are loaded from a clamped in-frame address; no
-     * in-limit candidate or filter tap ever reads them. */
-    constexpr int kWinWords = kWinRows * kWinDw; /* 552 */
-    constexpr int kPerLane = (kWinWords + 63) / 64;
-    uint32_t staged[kPerLane];
+    /* stage the window: all loads issued before any wait.  Lane (c4, rr) of
+     * lanes 0-59 loads dword c4 of rows rr, rr + 5, ...; window dwords
+     * outside the frame come from a clamped in-frame address (no in-limit
+     * candidate or filter tap reads them), rows past the window are loaded
+     * from a clamped row and dropped. */
+    constexpr int kStageIters = (kWinRows + 4) / 5; /* 10 */
+    const int c4 = lane % kWinDw, rr = lane / kWinDw;
+    const int fx = min(max(bx - 16 + 4 * c4, 0), W - 4);
+    uint32_t staged[kStageIters];
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
-        const int i = lane + 64 * j;
-        const int r = i / kWinDw, c4 = i - r * kWinDw;
-        const int fy = min(max(by - 15 + r, 0), H - 1);
-        const int fx = min(max(bx - 16 + 4 * c4, 0), W - 4);
-        staged[j] = *(const uint32_t *)(ref + (size_t)fy * W + fx);
+    for (int t = 0; t < kStageIters; t++) {
+        const int fy = min(max(by - 15 + rr + 5 * t, 0), H - 1);
+        staged[t] = *(const uint32_t *)(ref + (size_t)fy * W + fx);
     }
     const uint32_t cw = *(const uint32_t *)(cur + (size_t)(by + (lane >> 2)) * W + bx + 4 * (lane & 3));
 #pragma unroll
-    for (int j = 0; j < kPerLane; j++) {
-        const int i = lane + 64 * j;
-        if (i < kWinWords) win[i] = staged[j];
+    for (int t = 0; t < kStageIters; t++) {
+        const int r = rr + 5 * t;
+        if (lane < 5 * kWinDw && r < kWinRows) win[r * kPitch + c4] = staged[t];
     }
     if (kVariant == kMeSurface) curw[lane] = cw;
     __syncthreads();
@@ -280,10 +291,10 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
                 ty[i] = ci / 11;
                 const int vx = coarse_vec(tx[i]), vy = coarse_vec(ty[i]);
                 valid[i] = vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1 && lane + 64 * i < 121;
-                cand_addr(vx, vy, valid[i], off[i], sh[i]);
+                cand_addr<kPitch>(vx, vy, valid[i], off[i], sh[i]);
             }
             uint32_t acc[2];
-            sad16<2>(win, cur_mb, W, off, sh, acc);
+            sad16<2, kPitch>(win, cur_mb, W, off, sh, acc);
 #pragma unroll
             for (int i = 0; i < 2; i++) {
                 const uint32_t cost = valid[i] ? acc[i] : 65535u;
@@ -307,9 +318,9 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
             } else {
                 const bool valid = vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1;
                 int off[1], sh[1];
-                cand_addr(vx, vy, valid, off[0], sh[0]);
+                cand_addr<kPitch>(vx, vy, valid, off[0], sh[0]);
                 uint32_t acc[1];
-                sad16<1>(win, cur_mb, W, off, sh, acc);
+                sad16<1, kPitch>(win, cur_mb, W, off, sh, acc);
                 cost = valid ? acc[0] : 65535u;
             }
             best = lane < 49 ? (cost << 8) | morton((uint32_t)tx, (uint32_t)ty) : 0xffffffffu;
@@ -338,7 +349,7 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const int r = row_border ? wr : wr - 1 + i;
-        const uint32_t *w = win + r * kWinDw;
+        const uint32_t *w = win + r * kPitch;
         const uint32_t w0 = w[q], w1 = w[q + 1], w2 = w[q2];
         const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, bsh); /* pels -1, 0, 1, 2 */
         const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, bsh); /* pels 3, 4 */

@@ -110,7 +110,8 @@ typedef struct {
 
 /* max_streams: stream slots; hbm_budget: bytes of device memory the engine
  * may use for stream state (0 = no limit).  mbs_per_wave: MBs one wave of
- * the macroblock coder codes in sequence (0 = chosen by the autotuner). */
+ * the macroblock coder codes in sequence (0 = chosen per launch from the
+ * batch size). */
 H261G_API int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_budget, int32_t mbs_per_wave);
 H261G_API int h261g_engine_destroy(h261g_engine_t *e);
 
@@ -118,11 +119,14 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
 #define H261G_OPT_ME_VARIANT 1   /* 0: the reference's candidates, v_sad_u8 per candidate;
                                     1: full +-15 SAD surface by v_qsad_pk_u16_u8 (same selection,
                                     same results) */
-#define H261G_OPT_MBS_PER_WAVE 2 /* macroblocks one wave of the macroblock coder codes */
+#define H261G_OPT_MBS_PER_WAVE 2 /* macroblocks one wave of the macroblock coder codes
+                                    (0: chosen per launch from the batch size) */
 #define H261G_OPT_FUSED_TAILS 3  /* 1 (default): per-stream frame control runs in the tails of the
                                     two frame-step kernels; 0: as four separate launches */
-#define H261G_OPT_XCD_REMAP 4    /* 1: each XCD codes a contiguous run of macroblocks;
-                                    0 (default): workgroup i codes macroblock i */
+#define H261G_OPT_XCD_REMAP 4    /* bit 0: motion estimation, bit 1: macroblock coder; a set bit
+                                    gives each XCD a contiguous run of macroblocks (neighbours
+                                    share its L2), a clear bit has workgroup i code macroblock i.
+                                    Default 2. */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */
