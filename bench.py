@@ -46,22 +46,21 @@ ME_BYTES = {396: 352 * 288 * 2 + 44 * 396, 99: 176 * 144 * 2 + 44 * 99}
 MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 384 + 8)}
 
 
-# HBM bytes per macroblock actually moved by each kernel (FETCH_SIZE +
-# WRITE_SIZE per dispatch from two rocprofv3 --pmc passes over this
-# benchmark at --steps 8, each scaled by the same counter's reading for a
-# device copy of known size, divided by the dispatch's macroblock count).
-# Empty until measured: "traffic" is then null.
-TRAFFIC_PER_MB = {}
+# HBM bytes per macroblock actually moved by each kernel: FETCH_SIZE and
+# WRITE_SIZE per dispatch from two rocprofv3 --pmc passes over
+# `bench.py --steps 8 --no-cpu --no-plain --no-extra --copy-probe` (all 512
+# streams active, 126,720 MBs per dispatch), each scaled by what the same
+# counter reads for the 256 MiB device copy of --copy-probe, divided by the
+# dispatch's MB count (profiles/batched_512_streams_hbm_traffic.md: motion
+# estimation 506.3 MB read + 16.3 MB written, the coder 108.6 + 196.1 MB).
+TRAFFIC_PER_MB = {"me_kernel": (506.26e6 + 16.29e6) / 126720, "mb_kernel": (108.62e6 + 196.06e6) / 126720}
 
 
-def traffic_per_launch(kernel, algorithmic_bytes):
+def traffic_per_launch(kernel, mbs_per_launch):
     """HBM traffic of an average launch: measured bytes per MB times the
-    launch's MBs (the algorithmic bytes fix the MB count)"""
+    launch's MBs"""
     t = TRAFFIC_PER_MB.get(kernel)
-    if t is None:
-        return None
-    per_mb_alg = (ME_BYTES[396] if kernel == "me_kernel" else MB_BYTES[396]) / 396.0
-    return int(t * algorithmic_bytes / per_mb_alg)
+    return None if t is None else int(t * mbs_per_launch)
 
 
 def coarse_valid_candidates(mb_w, mb_h):
@@ -153,7 +152,7 @@ def run_job(eng, ids, specs, steps, start=0, compact_every=60, after_chunk=None)
 def algorithmic(specs, t0, t1):
     """algorithmic HBM bytes of the ME and MB kernels and byte-SADs of frame
     steps t0..t1-1 (intra frames: no reference read, no search)"""
-    me = mb = sad = 0
+    me = mb = sad = mbs = 0
     for s in specs:
         w, h = s["size"]
         nmb, fsz = w * h // 256, w * h * 3 // 2
@@ -163,7 +162,8 @@ def algorithmic(specs, t0, t1):
         me += inter * ME_BYTES[nmb] + intra * (w * h + 44 * nmb)
         mb += inter * MB_BYTES[nmb] + intra * (2 * fsz + nmb * (24 + 384 + 8))
         sad += inter * SAD_OPS[nmb]
-    return me, mb, sad
+        mbs += n * nmb
+    return me, mb, sad, mbs
 
 
 def plain_single_stream(Engine, frames=300, warmup=10):
@@ -326,9 +326,9 @@ def main():
 
     def kernel_figures(t, t0, t1):
         n = t["steps"]
-        me_b, mb_b, sad = algorithmic(specs, t0, t1)
+        me_b, mb_b, sad, mbs = algorithmic(specs, t0, t1)
         me_ms, mb_ms = t["me_ms"] / n, t["mb_ms"] / n
-        return dict(launches=n, me_ms=me_ms, mb_ms=mb_ms, me_bytes=me_b / n, mb_bytes=mb_b / n,
+        return dict(launches=n, me_ms=me_ms, mb_ms=mb_ms, me_bytes=me_b / n, mb_bytes=mb_b / n, mbs=mbs / n,
                     me_gbs=me_b / n / (me_ms * 1e-3) / 1e9, mb_gbs=mb_b / n / (mb_ms * 1e-3) / 1e9,
                     sad_rate=sad / n / (me_ms * 1e-3), other_ms=t["other_ms"] / n)
 
@@ -339,15 +339,18 @@ def main():
     achieved = job_k[key + "_gbs"]
     roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_per_launch(dominant, job_k[key + "_bytes"]),
+            "traffic": traffic_per_launch(dominant, job_k["mbs"]),
             "avg_launch_ms": round(job_k[key + "_ms"], 4),
             "algorithmic_bytes_per_launch": int(job_k[key + "_bytes"]), "launches": job_k["launches"]}
     for name, f in (("job_average", job_k), ("full_load", full_k)):
         roof[name] = {"launches": f["launches"],
+                      "mbs_per_launch": round(f["mbs"], 1),
                       "me_kernel": {"avg_launch_ms": round(f["me_ms"], 4), "hbm_gbs": round(f["me_gbs"], 1),
+                                    "traffic": traffic_per_launch("me_kernel", f["mbs"]),
                                     "sad_rate_tops": round(f["sad_rate"] / 1e12, 3),
                                     "sad_frac": round(f["sad_rate"] / SAD_PEAK, 4)},
                       "mb_kernel": {"avg_launch_ms": round(f["mb_ms"], 4), "hbm_gbs": round(f["mb_gbs"], 1),
+                                    "traffic": traffic_per_launch("mb_kernel", f["mbs"]),
                                     "hbm_frac": round(f["mb_gbs"] / HBM_PEAK_GBS, 4)},
                       "other_ms_per_step": round(f["other_ms"], 4)}
 

@@ -106,7 +106,7 @@ struct h261g_engine {
      * host keeps enqueueing ahead of the device and no host gap lands
      * between an event and the kernel it brackets */
     static constexpr int kEvRing = 64;
-    hipEvent_t ev[kEvRing][5] = {};
+    hipEvent_t ev[kEvRing][6] = {}; /* start/stop of ME, MB coder, frame tail */
     int64_t ev_recorded = 0, ev_read = 0;
     double me_ms = 0, mb_ms = 0, other_ms = 0;
     int64_t launches = 0;
@@ -189,7 +189,6 @@ int enqueue_step(h261g_engine *e, bool timed)
     const int n_mb = e->mb_used;
     hipStream_t st = e->stream;
     const int fused = e->fused_tails;
-    if (timed) (void)hipEventRecord(ev[0], st);
     MeLaunch ml{};
     ml.streams = e->desc;
     ml.mb_stream = e->mb_stream;
@@ -201,10 +200,8 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.variant = e->me_variant;
     ml.fuse_tail = fused;
     ml.xcd_remap = e->xcd_remap & 1;
-    if (launch_me(ml, st)) return H261G_ERR_DEVICE;
-    if (timed) (void)hipEventRecord(ev[1], st);
+    if (launch_me(ml, st, timed ? ev[0] : nullptr, timed ? ev[1] : nullptr)) return H261G_ERR_DEVICE;
     if (!fused && launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
-    if (timed) (void)hipEventRecord(ev[2], st);
     MbLaunch bl{};
     bl.streams = e->desc;
     bl.mb_stream = e->mb_stream;
@@ -219,19 +216,21 @@ int enqueue_step(h261g_engine *e, bool timed)
      * fewer statistic flushes) */
     bl.mbs_per_wave = e->mbs_per_wave ? e->mbs_per_wave : (n_mb >= 4 * kFillWaves ? 4 : (n_mb >= 2 * kFillWaves ? 2 : 1));
     bl.xcd_remap = (e->xcd_remap >> 1) & 1;
-    if (launch_mb(bl, st)) return H261G_ERR_DEVICE;
-    if (timed) (void)hipEventRecord(ev[3], st);
+    if (launch_mb(bl, st, timed ? ev[2] : nullptr, timed ? ev[3] : nullptr)) return H261G_ERR_DEVICE;
     if (fused) {
-        if (launch_frame_tail(e->desc, e->seq, e->dn, e->rc, e->side, ns, st)) return H261G_ERR_DEVICE;
+        if (launch_frame_tail(e->desc, e->seq, e->dn, e->rc, e->side, ns, st, timed ? ev[4] : nullptr,
+                              timed ? ev[5] : nullptr))
+            return H261G_ERR_DEVICE;
     } else {
+        /* separate kernels: "other" spans the three end-of-frame launches
+         * (rate-control begin, between the two big kernels, is not timed) */
+        if (timed) (void)hipEventRecord(ev[4], st);
         if (launch_denoise_update(e->dn, e->desc, ns, st)) return H261G_ERR_DEVICE;
         if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st)) return H261G_ERR_DEVICE;
         if (launch_advance(e->desc, e->seq, ns, st)) return H261G_ERR_DEVICE;
+        if (timed) (void)hipEventRecord(ev[5], st);
     }
-    if (timed) {
-        (void)hipEventRecord(ev[4], st);
-        e->ev_recorded++;
-    }
+    if (timed) e->ev_recorded++;
     return H261G_OK;
 }
 
@@ -239,14 +238,13 @@ int enqueue_step(h261g_engine *e, bool timed)
 int read_oldest_times(h261g_engine *e)
 {
     hipEvent_t *ev = e->ev[e->ev_read % h261g_engine::kEvRing];
-    if (!ok(hipEventSynchronize(ev[4]), "event")) return H261G_ERR_DEVICE;
-    float a = 0, b = 0, c = 0, f = 0;
+    if (!ok(hipEventSynchronize(ev[5]), "event")) return H261G_ERR_DEVICE;
+    float a = 0, c = 0, f = 0;
     (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-    (void)hipEventElapsedTime(&b, ev[1], ev[2]);
     (void)hipEventElapsedTime(&c, ev[2], ev[3]);
-    (void)hipEventElapsedTime(&f, ev[3], ev[4]);
+    (void)hipEventElapsedTime(&f, ev[4], ev[5]);
     e->me_ms += a;
-    e->other_ms += b + f;
+    e->other_ms += f;
     e->mb_ms += c;
     e->launches++;
     e->ev_read++;
@@ -341,7 +339,7 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipMemset(e->desc, 0, sizeof(StreamDesc) * max_streams), "hipMemset") &&
                 ok(hipMemset(e->seq, 0, sizeof(SeqState) * max_streams), "hipMemset");
     for (int r = 0; good && r < h261g_engine::kEvRing; r++)
-        for (int i = 0; good && i < 5; i++) good = ok(hipEventCreate(&e->ev[r][i]), "event");
+        for (int i = 0; good && i < 6; i++) good = ok(hipEventCreate(&e->ev[r][i]), "event");
     if (!good) {
         h261g_engine_destroy(e);
         return H261G_ERR_DEVICE;
@@ -372,7 +370,7 @@ int h261g_engine_destroy(h261g_engine_t *e)
     hipFree(e->levels);
     hipHostFree(e->mbstruct);
     for (int r = 0; r < h261g_engine::kEvRing; r++)
-        for (int i = 0; i < 5; i++)
+        for (int i = 0; i < 6; i++)
             if (e->ev[r][i]) (void)hipEventDestroy(e->ev[r][i]);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;

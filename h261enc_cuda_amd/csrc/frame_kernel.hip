@@ -4,6 +4,7 @@
  * per stream) of the engine's default mode, and the separate kernels of its
  * unfused mode and of the legacy path's denoise update.
  */
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "frame_control.h"
@@ -76,11 +77,22 @@ __global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
 }  // namespace
 
 int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
-                      int n_streams, void *stream)
+                      int n_streams, void *stream, void *ev_start, void *ev_stop)
 {
-    if (n_streams <= 0) return 0;
-    hipLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, streams, seq, dn, rc,
-                       side, nullptr);
+    if (n_streams <= 0) {
+        if (ev_start) { /* nothing to launch: an empty timed interval */
+            (void)hipEventRecord((hipEvent_t)ev_start, (hipStream_t)stream);
+            (void)hipEventRecord((hipEvent_t)ev_stop, (hipStream_t)stream);
+        }
+        return 0;
+    }
+    int32_t *no_bytes = nullptr;
+    if (ev_start)
+        hipExtLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream,
+                              (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0, streams, seq, dn, rc, side, no_bytes);
+    else
+        hipLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, streams, seq, dn, rc,
+                           side, no_bytes);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -149,7 +149,9 @@ enum : int32_t {
     kMeTwoStage = 0, /* the reference's candidates only, one v_sad_u8 lane per candidate */
     kMeSurface = 1,  /* full +-15 SAD surface by v_qsad_pk_u16_u8, then the same selection */
 };
-int launch_me(const MeLaunch &p, void *stream);
+/* ev_start / ev_stop (hipEvent_t, optional): timestamps of the dispatch
+ * itself (hipExtLaunchKernelGGL), for profiling; never under capture */
+int launch_me(const MeLaunch &p, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
 
 /* one wave per MB: MC/loop filter, DCT, denoise, quant, dequant, IDCT,
  * reconstruction; statistics for the denoise update; bit counts. */
@@ -166,7 +168,7 @@ struct MbLaunch {
     int32_t mbs_per_wave;         /* MBs coded by one wave in sequence */
     int32_t xcd_remap;            /* 1: contiguous macroblock runs per XCD (xcd_logical) */
 };
-int launch_mb(const MbLaunch &p, void *stream);
+int launch_mb(const MbLaunch &p, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
 
 /* per stream, 64 lanes: new denoise offsets from this frame's statistics,
  * statistics reset (src/mbgpu_kernels.cuh:478-503) */
@@ -184,7 +186,7 @@ int launch_advance(StreamDesc *streams, SeqState *seq, int n_streams, void *stre
 /* per stream, one wave: denoise update, frame size, rate-control update and
  * advance in one launch */
 int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
-                      int n_streams, void *stream);
+                      int n_streams, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
 
 }  // namespace h261g
 

@@ -23,6 +23,7 @@
  * workgroup); the CBP is the OR of this frame's coded blocks, the intended
  * result of the reference's racy reset/atomicOr (mbgpu:676-721).
  */
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "h261_vlc.h"
@@ -527,14 +528,24 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 
 }  // namespace
 
-int launch_mb(const MbLaunch &p, void *stream)
+int launch_mb(const MbLaunch &p, void *stream, void *ev_start, void *ev_stop)
 {
-    if (p.n_mb <= 0) return 0;
+    if (p.n_mb <= 0) {
+        if (ev_start) { /* nothing to launch: an empty timed interval */
+            (void)hipEventRecord((hipEvent_t)ev_start, (hipStream_t)stream);
+            (void)hipEventRecord((hipEvent_t)ev_stop, (hipStream_t)stream);
+        }
+        return 0;
+    }
     const int per = p.mbs_per_wave > 0 ? p.mbs_per_wave : 1;
     MbLaunch q = p;
     q.mbs_per_wave = per;
     const int grid = launch_grid((p.n_mb + per - 1) / per, p.xcd_remap);
-    hipLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, q);
+    if (ev_start)
+        hipExtLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, (hipEvent_t)ev_start,
+                              (hipEvent_t)ev_stop, 0, q);
+    else
+        hipLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, q);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

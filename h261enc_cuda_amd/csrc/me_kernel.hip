@@ -20,6 +20,7 @@
  * Out-of-limit candidates cost 65535 (megpu:67-92); limits keep every
  * fetch inside the frame (gpu_device.cu:226-265).
  */
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "frame_control.h"
@@ -441,13 +442,22 @@ __global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
 
 }  // namespace
 
-int launch_me(const MeLaunch &p, void *stream)
+int launch_me(const MeLaunch &p, void *stream, void *ev_start, void *ev_stop)
 {
-    if (p.n_mb <= 0) return 0;
-    if (p.variant == kMeSurface)
-        hipLaunchKernelGGL(me_kernel<kMeSurface>, dim3(launch_grid(p.n_mb, p.xcd_remap)), dim3(64), 0, (hipStream_t)stream, p);
+    if (p.n_mb <= 0) {
+        if (ev_start) { /* nothing to launch: an empty timed interval */
+            (void)hipEventRecord((hipEvent_t)ev_start, (hipStream_t)stream);
+            (void)hipEventRecord((hipEvent_t)ev_stop, (hipStream_t)stream);
+        }
+        return 0;
+    }
+    const dim3 grid(launch_grid(p.n_mb, p.xcd_remap));
+    auto k = p.variant == kMeSurface ? me_kernel<kMeSurface> : me_kernel<kMeTwoStage>;
+    if (ev_start)
+        hipExtLaunchKernelGGL(k, grid, dim3(64), 0, (hipStream_t)stream, (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0,
+                              p);
     else
-        hipLaunchKernelGGL(me_kernel<kMeTwoStage>, dim3(launch_grid(p.n_mb, p.xcd_remap)), dim3(64), 0, (hipStream_t)stream, p);
+        hipLaunchKernelGGL(k, grid, dim3(64), 0, (hipStream_t)stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
