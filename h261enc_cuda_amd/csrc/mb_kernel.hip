@@ -239,6 +239,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
          * dwords, and this lane's eight denoise offsets */
         const int t_off = intra ? 0 : 1;
         uint32_t offw[8];
+        uint32_t offp[4]; /* the offsets as UInt16 (mbgpu:525-528), two per register */
         int pred[8], res[8];
         uint2 cw = make_uint2(0, 0);
         uint32_t rw[3][3];
@@ -247,6 +248,8 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             const int32_t *offs = t_off == 0 ? p.denoise[s].intra_offset : p.denoise[s].inter_offset;
 #pragma unroll
             for (int r = 0; r < 8; r++) offw[r] = (uint32_t)offs[rc + 8 * r];
+#pragma unroll
+            for (int r = 0; r < 4; r++) offp[r] = (offw[2 * r] & 0xffffu) | (offw[2 * r + 1] << 16);
             cw = *(const uint2 *)(d.cur[plane] + (size_t)(py + rc) * stride + px);
             const int rx = px + mvx;
             const int a = rx & ~3;
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 #pragma unroll
             for (int r = 0; r < 8; r++) {
                 int i = c + 8 * r;
-                int off = (int)(uint16_t)offw[r];
+                const int off = (int)((offp[r >> 1] >> (16 * (r & 1))) & 0xffffu);
                 int cc = coef[r];
                 int a = cc < 0 ? -cc : cc;
                 atomicAdd(&stat[t][i], a);

@@ -213,7 +213,7 @@ __device__ __forceinline__ void sad_surface(const uint32_t *win, const uint32_t 
 }
 
 template <int kVariant>
-__global__ __launch_bounds__(64) void me_kernel(MeLaunch p)
+__global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
 {
     constexpr int kPitch = win_pitch<kVariant>();
     __shared__ __align__(16) uint32_t win[kWinRows * kPitch];
