@@ -153,44 +153,101 @@ __device__ __forceinline__ int tcoeff_bits(const uint8_t *tlen, int run, int lev
     return (run < kTcoeffRuns && a < kTcoeffLevels) ? (int)tlen[run * kTcoeffLevels + a] : kEscapeBits;
 }
 
-/* flushes this wave's denoise statistics of stream s (LDS -> global
- * atomics); the frame tail reads them in a later launch */
-__device__ __forceinline__ void flush_segment(const MbLaunch &p, int s, int32_t (*stat)[64], int32_t *scount, int lane)
+/* Per-lane view of the block a lane works on in one pass: lane (k, r) of a
+ * pass takes block 8 * pass + k of its group of G macroblocks (6 blocks
+ * each), row or column r. */
+struct LaneBlock {
+    bool act;      /* the block exists and its stream is active */
+    int g, s;      /* global MB, stream */
+    int b;         /* block in the MB: 0-3 luma, 4 Cb, 5 Cr */
+    int slot;      /* statistics slot: 0 = the group's first stream, 1 = another */
+    bool intra, filter;
+    int quant, t;  /* t: 0 intra, 1 inter (statistics table) */
+    int px, py, stride, mvx, mvy, plane;
+};
+
+__device__ __forceinline__ LaneBlock lane_block(const MbLaunch &p, int g, int b, int s0)
 {
-    __syncthreads();
-    DenoiseState *dn = p.denoise + s;
-    if (stat[0][lane]) atomicAdd(&dn->intra_noise[lane], stat[0][lane]);
-    if (stat[1][lane]) atomicAdd(&dn->inter_noise[lane], stat[1][lane]);
-    if (lane == 0 && scount[0]) atomicAdd(&dn->intra_count, scount[0]);
-    if (lane == 1 && scount[1]) atomicAdd(&dn->inter_count, scount[1]);
-    __syncthreads();
-    stat[0][lane] = 0;
-    stat[1][lane] = 0;
-    if (lane < 2) scount[lane] = 0;
-    __syncthreads();
+    LaneBlock L{};
+    L.g = g;
+    L.b = b;
+    L.s = g < p.n_mb ? p.mb_stream[g] : -1;
+    L.act = L.s >= 0;
+    const StreamDesc &d = p.streams[L.act ? L.s : 0];
+    L.act = L.act && (d.flags & kStreamActive);
+    if (!L.act) return L;
+    L.slot = L.s == s0 ? 0 : 1;
+    const MbMode md = p.mode[g];
+    const int m = g - d.mb_base;
+    const int bx = (m % d.mb_w) * 16, by = (m / d.mb_w) * 16;
+    const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[L.s].pic_quant : d.pic_quant;
+    L.quant = clampi(picq + md.quant_adjust, 1, 31); /* gpu_device.cu:625-627 */
+    L.intra = md.type == H261_MB_TYPE_INTRA;
+    L.filter = (md.flags & H261_MB_FILTER) != 0;
+    L.t = L.intra ? 0 : 1;
+    L.plane = b < 4 ? 0 : b - 3;
+    if (b < 4) {
+        L.px = bx + (b & 1) * 8;
+        L.py = by + ((b >> 1) & 1) * 8;
+        L.stride = d.width;
+        L.mvx = md.mvx;
+        L.mvy = md.mvy;
+    } else { /* chroma vector = luma vector / 2, truncated (mbgpu:266-269) */
+        L.px = bx >> 1;
+        L.py = by >> 1;
+        L.stride = d.width >> 1;
+        L.mvx = md.mvx / 2;
+        L.mvy = md.mvy / 2;
+    }
+    return L;
 }
 
+/* stream of the first MB of [g0, g0 + n) that belongs to an active stream,
+ * or -1 */
+__device__ __forceinline__ int first_stream(const MbLaunch &p, int g0, int n)
+{
+    for (int i = 0; i < n && g0 + i < p.n_mb; i++) {
+        const int s = p.mb_stream[g0 + i];
+        if (s >= 0 && (p.streams[s].flags & kStreamActive)) return s;
+    }
+    return -1;
+}
+
+/* flushes the wave's denoise statistics of one stream (LDS -> global
+ * atomics) and clears them; the frame tail reads them in a later launch */
+__device__ __forceinline__ void flush_stats(const MbLaunch &p, int s, int32_t (*st)[64], int32_t *cnt, int lane)
+{
+    DenoiseState *dn = p.denoise + s;
+    if (st[0][lane]) atomicAdd(&dn->intra_noise[lane], st[0][lane]);
+    if (st[1][lane]) atomicAdd(&dn->inter_noise[lane], st[1][lane]);
+    if (lane == 0 && cnt[0]) atomicAdd(&dn->intra_count, cnt[0]);
+    if (lane == 1 && cnt[1]) atomicAdd(&dn->inter_count, cnt[1]);
+}
+
+/* One wave codes groups of G consecutive MBs (G = 1: one pass of 6 blocks,
+ * lanes 48-63 idle; G = 4: 24 blocks in three full passes of 8).  Per pass:
+ * phase 1 lane (block, row), phase 2 lane (block, column), phase 3 lane
+ * (block, row); then per MB of the group the CBP, bit count and outputs. */
+template <int G>
 __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 {
-    __shared__ float tf[6 * kBlkPitch];
-    __shared__ __align__(16) int32_t lv[6 * 64];
-    __shared__ int32_t stat[2][64];
-    __shared__ int32_t scount[2];
-    __shared__ int32_t sblk[8];
+    constexpr int kBlocks = 6 * G;
+    constexpr int kPasses = (kBlocks + 7) / 8;
+    __shared__ float tf[8 * kBlkPitch];
+    __shared__ __align__(16) int16_t lv[kBlocks * 64];
+    __shared__ int32_t stat[2][2][64]; /* [stream slot][intra, inter][coefficient] */
+    __shared__ int32_t scount[2][2];
     __shared__ __align__(16) uint32_t tlen32[kTlenWords];
     const uint8_t *tlen = (const uint8_t *)tlen32;
 
     const int lane = threadIdx.x;
-    const int b = lane >> 3; /* block 0-5 (6, 7 idle) */
-    const int rc = lane & 7; /* row in phases 1/3, column in phase 2 */
-    const bool act = b < 6;
+    const int k8 = lane >> 3; /* block of the pass */
+    const int rc = lane & 7;  /* row in phases 1/3, column in phase 2 */
 
-    stat[0][lane] = 0;
-    stat[1][lane] = 0;
-    if (lane < 2) scount[lane] = 0;
+    stat[0][0][lane] = stat[0][1][lane] = stat[1][0][lane] = stat[1][1][lane] = 0;
+    if (lane < 4) scount[lane >> 1][lane & 1] = 0;
     if (lane < kTlenWords / 2) ((uint2 *)tlen32)[lane] = ((const uint2 *)c_tlen.w)[lane];
-    /* per-lane constants of the loop: zig-zag position of lane, column
-     * scale factors of lane (b, c) */
+    /* per-lane constants: zig-zag position of lane, column scale factors */
     const int zz = c_zigzag[lane];
     const float post_c = (float)((double)c_fwd_post[rc] * 0.125);
     const float pre_c = c_inv_pre[rc];
@@ -198,335 +255,330 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     const int n_wg = (p.n_mb + p.mbs_per_wave - 1) / p.mbs_per_wave;
     const int wg = launch_index(blockIdx.x, n_wg, p.xcd_remap);
     if (wg >= n_wg) return;
-    for (int it = 0; it < p.mbs_per_wave; it++) {
-        const int g = wg * p.mbs_per_wave + it;
-        if (g >= p.n_mb) break;
-        const int s = p.mb_stream[g];
-        if (s < 0) continue; /* unmapped range of a closed stream */
-        const StreamDesc &d = p.streams[s];
-        if (!(d.flags & kStreamActive)) continue;
-        const int m = g - d.mb_base;
-        const MbMode md = p.mode[g];
-        const int W = d.width;
-        const int bx = (m % d.mb_w) * 16, by = (m / d.mb_w) * 16;
-        const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[s].pic_quant : d.pic_quant;
-        const int quant = clampi(picq + md.quant_adjust, 1, 31); /* gpu_device.cu:625-627 */
-        const bool intra = md.type == H261_MB_TYPE_INTRA;
-        const bool filter = (md.flags & H261_MB_FILTER) != 0;
-        const int t = intra ? 0 : 1;
-        (void)t;
+    for (int grp = 0; grp * G < p.mbs_per_wave; grp++) {
+        const int g0 = wg * p.mbs_per_wave + grp * G;
+        if (g0 >= p.n_mb) break;
+        const int s0 = first_stream(p, g0, G);
+        if (s0 < 0) continue;
+        int s1 = -1; /* the group's other stream, if any */
+        uint32_t coded_bits = 0;
+        __syncthreads();
 
-        /* ---- phase 1: lane (b, r) ---- */
-        const int plane = b < 4 ? 0 : b - 3;
-        int px, py, stride, mvx, mvy;
-        if (b < 4) {
-            px = bx + (b & 1) * 8;
-            py = by + ((b >> 1) & 1) * 8;
-            stride = W;
-            mvx = md.mvx;
-            mvy = md.mvy;
-        } else { /* chroma vector = luma vector / 2, truncated (mbgpu:266-269) */
-            px = bx >> 1;
-            py = by >> 1;
-            stride = W >> 1;
-            mvx = md.mvx / 2;
-            mvy = md.mvy / 2;
-        }
-        /* issue every global load of the MB before using any: the current
-         * row, three reference rows (rc-1, rc, rc+1 clamped to the block --
-         * the loop filter is 1-D at the block edges; an intra MB has a zero
-         * vector, so its addresses are in the frame as well) as aligned
-         * dwords, and this lane's eight denoise offsets */
-        const int t_off = intra ? 0 : 1;
-        uint32_t offw[8];
-        uint32_t offp[4]; /* the offsets as UInt16 (mbgpu:525-528), two per register */
-        int pred[8], res[8];
-        uint2 cw = make_uint2(0, 0);
-        uint32_t rw[3][3];
-        int o = 0;
-        if (act) {
-            const int32_t *offs = t_off == 0 ? p.denoise[s].intra_offset : p.denoise[s].inter_offset;
+#pragma unroll 1
+        for (int pass = 0; pass < kPasses; pass++) {
+            const int bi = pass * 8 + k8; /* block of the group */
+            const int mi = bi / 6;
+            const LaneBlock L = lane_block(p, bi < kBlocks ? g0 + mi : p.n_mb, bi - 6 * mi, s0);
+            if (L.act && L.slot) s1 = L.s;
+
+            /* ---- phase 1: loads first (current row, reference rows rc-1..rc+1
+             * clamped to the block -- the loop filter is 1-D at the block
+             * edges; an intra MB has a zero vector -- as aligned dwords, and
+             * this lane's eight denoise offsets), then MC / loop filter,
+             * residual, DCT rows ---- */
+            uint32_t offp[4]; /* the offsets as UInt16 (mbgpu:525-528), two per register */
+            uint32_t predw[2] = {0, 0};
+            if (L.act) {
+                const StreamDesc &d = p.streams[L.s];
+                const int32_t *offs = L.t == 0 ? p.denoise[L.s].intra_offset : p.denoise[L.s].inter_offset;
+                uint32_t offw[8];
 #pragma unroll
-            for (int r = 0; r < 8; r++) offw[r] = (uint32_t)offs[rc + 8 * r];
-#pragma unroll
-            for (int r = 0; r < 4; r++) offp[r] = (offw[2 * r] & 0xffffu) | (offw[2 * r + 1] << 16);
-            cw = *(const uint2 *)(d.cur[plane] + (size_t)(py + rc) * stride + px);
-            const int rx = px + mvx;
-            const int a = rx & ~3;
-            o = rx & 3;
-            /* the third dword is needed only when o > 0, and then it lies in the row */
-            const int a2 = a + 8 <= stride - 4 ? a + 8 : a;
-            const uint8_t *rbase = d.ref[plane] + (size_t)(py + mvy) * stride;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int rr = clampi(rc + k - 1, 0, 7);
-                const uint8_t *q = rbase + (size_t)rr * stride;
-                rw[k][0] = *(const uint32_t *)(q + a);
-                rw[k][1] = *(const uint32_t *)(q + a + 4);
-                rw[k][2] = *(const uint32_t *)(q + a2);
-            }
-        }
-        if (act) {
-            int cpel[8];
-#pragma unroll
-            for (int x = 0; x < 4; x++) {
-                cpel[x] = (cw.x >> (8 * x)) & 0xff;
-                cpel[x + 4] = (cw.y >> (8 * x)) & 0xff;
-            }
-            /* the 8 reference pels of rows rc-1, rc, rc+1 */
-            int a8[3][8];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const uint32_t lo = __builtin_amdgcn_alignbyte(rw[k][1], rw[k][0], o);
-                const uint32_t hi = __builtin_amdgcn_alignbyte(rw[k][2], rw[k][1], o);
-#pragma unroll
-                for (int x = 0; x < 4; x++) {
-                    a8[k][x] = (lo >> (8 * x)) & 0xff;
-                    a8[k][x + 4] = (hi >> (8 * x)) & 0xff;
-                }
-            }
-            if (intra) {
-#pragma unroll
-                for (int x = 0; x < 8; x++) pred[x] = 0;
-            } else if (!filter) {
-#pragma unroll
-                for (int x = 0; x < 8; x++) pred[x] = a8[1][x];
-            } else {
-                /* H.261 loop filter, separable [1 2 1]: 1-D at the block
-                 * edges, corners unfiltered (mbgpu:79-150) */
-                int h[3][8];
+                for (int r = 0; r < 8; r++) offw[r] = (uint32_t)offs[rc + 8 * r];
+                const uint2 cw = *(const uint2 *)(d.cur[L.plane] + (size_t)(L.py + rc) * L.stride + L.px);
+                const int rx = L.px + L.mvx;
+                const int a = rx & ~3, o = rx & 3;
+                /* the third dword is needed only when o > 0, and then it lies in the row */
+                const int a2 = a + 8 <= L.stride - 4 ? a + 8 : a;
+                const uint8_t *rbase = d.ref[L.plane] + (size_t)(L.py + L.mvy) * L.stride;
+                uint32_t rw[3][3];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    h[k][0] = 4 * a8[k][0];
-#pragma unroll
-                    for (int x = 1; x < 7; x++) h[k][x] = a8[k][x - 1] + 2 * a8[k][x] + a8[k][x + 1];
-                    h[k][7] = 4 * a8[k][7];
+                    const int rr = clampi(rc + k - 1, 0, 7);
+                    const uint8_t *q = rbase + (size_t)rr * L.stride;
+                    rw[k][0] = *(const uint32_t *)(q + a);
+                    rw[k][1] = *(const uint32_t *)(q + a + 4);
+                    rw[k][2] = *(const uint32_t *)(q + a2);
                 }
-                if (rc == 0 || rc == 7) {
 #pragma unroll
-                    for (int x = 0; x < 8; x++) pred[x] = (h[1][x] + 2) >> 2;
-                } else {
+                for (int r = 0; r < 4; r++) offp[r] = (offw[2 * r] & 0xffffu) | (offw[2 * r + 1] << 16);
+                /* the 8 reference pels of rows rc-1, rc, rc+1 */
+                int a8[3][8];
 #pragma unroll
-                    for (int x = 0; x < 8; x++) pred[x] = (h[0][x] + 2 * h[1][x] + h[2][x] + 8) >> 4;
-                }
-            }
+                for (int k = 0; k < 3; k++) {
+                    const uint32_t lo = __builtin_amdgcn_alignbyte(rw[k][1], rw[k][0], o);
+                    const uint32_t hi = __builtin_amdgcn_alignbyte(rw[k][2], rw[k][1], o);
 #pragma unroll
-            for (int x = 0; x < 8; x++) res[x] = cpel[x] - pred[x];
-
-            /* forward DCT, row pass */
-            float o[8];
-            fdct_core((float)(res[0] + res[7]), (float)(res[1] + res[6]), (float)(res[2] + res[5]),
-                      (float)(res[3] + res[4]), (float)(res[3] - res[4]), (float)(res[2] - res[5]),
-                      (float)(res[1] - res[6]), (float)(res[0] - res[7]), o);
-#pragma unroll
-            for (int k = 0; k < 8; k++) tf[b * kBlkPitch + rc * 8 + k] = o[k];
-        }
-        __syncthreads();
-
-        /* ---- phase 2: lane (b, c) ---- */
-        int coef[8];
-        int lvl[8];
-        int blk_accum = 0;
-        if (act) {
-            const int c = rc;
-            float v[8];
-#pragma unroll
-            for (int r = 0; r < 8; r++) v[r] = tf[b * kBlkPitch + r * 8 + c];
-            const float cs = post_c;
-            float o[8];
-            fdct_core(v[0] + v[7], v[1] + v[6], v[2] + v[5], v[3] + v[4], v[3] - v[4], v[2] - v[5], v[1] - v[6],
-                      v[0] - v[7], o);
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                coef[k] = (int)floorf((float)((double)(cs * kFwdPost[k] * o[k]) + 0.49999));
-
-            /* denoise with the previous frame's offsets, staged as UInt16
-             * (mbgpu:525-528, 586-604) */
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                int i = c + 8 * r;
-                const int off = (int)((offp[r >> 1] >> (16 * (r & 1))) & 0xffffu);
-                int cc = coef[r];
-                int a = cc < 0 ? -cc : cc;
-                atomicAdd(&stat[t][i], a);
-                a = a - off;
-                a = a > 0 ? a : 0;
-                coef[r] = cc < 0 ? -a : a;
-            }
-
-            /* forward quantiser (mbgpu:685-716) */
-            const float inv = c_inv_quant[quant];
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                int i = c + 8 * r;
-                int l;
-                if (intra && i == 0) {
-                    l = coef[r] / 8;
-                    l = clampi(l, 1, 254);
-                    if (l == 128) l = 255;
-                    blk_accum += 100;
-                } else {
-                    float f = (float)coef[r] * inv;
-                    l = clampi((int)f, -127, 127);
-                    int al = l < 0 ? -l : l;
-                    blk_accum += al > 1 ? 100 : al;
-                }
-                lvl[r] = l;
-                lv[b * 64 + i] = l;
-            }
-        }
-        /* sum the 8 columns of the block */
-        blk_accum += __shfl_xor(blk_accum, 1, 64);
-        blk_accum += __shfl_xor(blk_accum, 2, 64);
-        blk_accum += __shfl_xor(blk_accum, 4, 64);
-        const bool coded = act && blk_accum > 2;
-        /* cbp of the MB: ballot of the lanes c == 0 of coded blocks */
-        const uint64_t coded_mask = __ballot(coded && rc == 0);
-        int cbp = 0;
-#pragma unroll
-        for (int k = 0; k < 6; k++)
-            if ((coded_mask >> (8 * k)) & 1) cbp |= kCbpBit[k];
-
-        if (act) {
-            const int c = rc;
-            /* dequantiser (mbgpu:763-814); an uncoded block is all zero */
-            const int qadd = coded ? ((quant - 1) | 1) : 0, qmul = coded ? quant * 2 : 0;
-            int deq[8];
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                int i = c + 8 * r;
-                int l = lvl[r];
-                if (intra && i == 0) {
-                    deq[r] = l == 255 ? 1024 : l * 8;
-                } else {
-                    int x = l * qmul;
-                    if (x > 0)
-                        x += qadd;
-                    else if (x < 0)
-                        x -= qadd;
-                    deq[r] = clampi(x, -2048, 2047);
-                }
-            }
-            /* IDCT prescale and column pass (mbgpu:836-888) */
-            float tv[8], o[8];
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                const float cs = (float)((double)kInvPre[r] * 0.125);
-                tv[r] = ((float)deq[r] * pre_c) * cs;
-            }
-            idct_core(tv, o);
-            __builtin_amdgcn_s_barrier(); /* all phase-2 reads of tf are done */
-#pragma unroll
-            for (int r = 0; r < 8; r++) tf[b * kBlkPitch + r * 8 + c] = o[r];
-        } else {
-            __builtin_amdgcn_s_barrier();
-        }
-        __syncthreads();
-
-        /* ---- phase 3: lane (b, r) ---- */
-        if (act) {
-            float v[8], o[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = tf[b * kBlkPitch + rc * 8 + k];
-            idct_core(v, o);
-            uint32_t w0 = 0, w1 = 0;
-#pragma unroll
-            for (int x = 0; x < 8; x++) {
-                int rres = clampi((int)floor((double)o[x] + 0.49999), -256, 255);
-                int val = clampi(pred[x] + rres, 0, 255);
-                if (x < 4)
-                    w0 |= (uint32_t)val << (8 * x);
-                else
-                    w1 |= (uint32_t)val << (8 * (x - 4));
-            }
-            *(uint2 *)(d.rec[plane] + (size_t)(py + rc) * stride + px) = make_uint2(w0, w1);
-        }
-
-        /* ---- bit count of the MB's CBP + TCOEFF layer ---- */
-        int bits = 0;
-        const bool written_any = intra || cbp;
-        if (p.side && written_any) {
-            /* lane k = transmission position k of each block in turn */
-            for (int k = 0; k < 6; k++) {
-                const bool blk_written = intra || (cbp & kCbpBit[k]);
-                if (!blk_written) continue;
-                int l = lv[k * 64 + zz];
-                uint64_t nz = __ballot(l != 0);
-                if (l != 0) {
-                    uint64_t below = lane ? (nz & ((~0ull) >> (64 - lane))) : 0ull;
-                    int prev = below ? 63 - __builtin_clzll(below) : -1;
-                    int run = lane - prev - 1;
-                    if (lane == 0) {
-                        if (intra)
-                            bits += 8; /* DC: 8-bit fixed length */
-                        else if (l == 1 || l == -1)
-                            bits += 2; /* first coefficient '1s' */
-                        else
-                            bits += tcoeff_bits(tlen, 0, l);
-                    } else {
-                        bits += tcoeff_bits(tlen, run, l);
+                    for (int x = 0; x < 4; x++) {
+                        a8[k][x] = (lo >> (8 * x)) & 0xff;
+                        a8[k][x + 4] = (hi >> (8 * x)) & 0xff;
                     }
                 }
-                if (lane == 0) bits += 2; /* EOB */
-            }
+                int pred[8];
+                if (L.intra) {
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) bits += __shfl_xor(bits, off, 64);
-            if (!intra) bits += c_cbp[cbp].len;
+                    for (int x = 0; x < 8; x++) pred[x] = 0;
+                } else if (!L.filter) {
+#pragma unroll
+                    for (int x = 0; x < 8; x++) pred[x] = a8[1][x];
+                } else {
+                    /* H.261 loop filter, separable [1 2 1]: 1-D at the block
+                     * edges, corners unfiltered (mbgpu:79-150) */
+                    int h[3][8];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        h[k][0] = 4 * a8[k][0];
+#pragma unroll
+                        for (int x = 1; x < 7; x++) h[k][x] = a8[k][x - 1] + 2 * a8[k][x] + a8[k][x + 1];
+                        h[k][7] = 4 * a8[k][7];
+                    }
+                    if (rc == 0 || rc == 7) {
+#pragma unroll
+                        for (int x = 0; x < 8; x++) pred[x] = (h[1][x] + 2) >> 2;
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < 8; x++) pred[x] = (h[0][x] + 2 * h[1][x] + h[2][x] + 8) >> 4;
+                    }
+                }
+                int res[8];
+#pragma unroll
+                for (int x = 0; x < 4; x++) {
+                    res[x] = (int)((cw.x >> (8 * x)) & 0xff) - pred[x];
+                    res[x + 4] = (int)((cw.y >> (8 * x)) & 0xff) - pred[x + 4];
+                    predw[0] |= (uint32_t)pred[x] << (8 * x);
+                    predw[1] |= (uint32_t)pred[x + 4] << (8 * x);
+                }
+                /* forward DCT, row pass */
+                float o8[8];
+                fdct_core((float)(res[0] + res[7]), (float)(res[1] + res[6]), (float)(res[2] + res[5]),
+                          (float)(res[3] + res[4]), (float)(res[3] - res[4]), (float)(res[2] - res[5]),
+                          (float)(res[1] - res[6]), (float)(res[0] - res[7]), o8);
+#pragma unroll
+                for (int k = 0; k < 8; k++) tf[k8 * kBlkPitch + rc * 8 + k] = o8[k];
+            }
+            __syncthreads();
+
+            /* ---- phase 2: lane (block, column c) ---- */
+            int lvl[8];
+            int blk_accum = 0;
+            if (L.act) {
+                const int c = rc;
+                float v[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) v[r] = tf[k8 * kBlkPitch + r * 8 + c];
+                const float cs = post_c;
+                float o8[8];
+                fdct_core(v[0] + v[7], v[1] + v[6], v[2] + v[5], v[3] + v[4], v[3] - v[4], v[2] - v[5],
+                          v[1] - v[6], v[0] - v[7], o8);
+                int coef[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    coef[k] = (int)floorf((float)((double)(cs * kFwdPost[k] * o8[k]) + 0.49999));
+
+                /* denoise with the previous frame's offsets, staged as UInt16
+                 * (mbgpu:525-528, 586-604) */
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    const int i = c + 8 * r;
+                    const int off = (int)((offp[r >> 1] >> (16 * (r & 1))) & 0xffffu);
+                    const int cc = coef[r];
+                    int av = cc < 0 ? -cc : cc;
+                    atomicAdd(&stat[L.slot][L.t][i], av);
+                    av = av - off;
+                    av = av > 0 ? av : 0;
+                    coef[r] = cc < 0 ? -av : av;
+                }
+
+                /* forward quantiser (mbgpu:685-716) */
+                const float inv = c_inv_quant[L.quant];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    const int i = c + 8 * r;
+                    int l;
+                    if (L.intra && i == 0) {
+                        l = coef[r] / 8;
+                        l = clampi(l, 1, 254);
+                        if (l == 128) l = 255;
+                        blk_accum += 100;
+                    } else {
+                        const float f = (float)coef[r] * inv;
+                        l = clampi((int)f, -127, 127);
+                        const int al = l < 0 ? -l : l;
+                        blk_accum += al > 1 ? 100 : al;
+                    }
+                    lvl[r] = l;
+                    lv[bi * 64 + i] = (int16_t)l;
+                }
+            }
+            /* sum the 8 columns of the block */
+            blk_accum += __shfl_xor(blk_accum, 1, 64);
+            blk_accum += __shfl_xor(blk_accum, 2, 64);
+            blk_accum += __shfl_xor(blk_accum, 4, 64);
+            const bool coded = L.act && blk_accum > 2;
+            /* coded blocks of the pass: ballot of the lanes c == 0 */
+            const uint64_t coded_mask = __ballot(coded && rc == 0);
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if ((coded_mask >> (8 * k)) & 1) coded_bits |= 1u << (pass * 8 + k);
+
+            if (L.act) {
+                const int c = rc;
+                /* dequantiser (mbgpu:763-814); an uncoded block is all zero */
+                const int qadd = coded ? ((L.quant - 1) | 1) : 0, qmul = coded ? L.quant * 2 : 0;
+                float tv[8], o8[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    const int i = c + 8 * r;
+                    const int l = lvl[r];
+                    int dq;
+                    if (L.intra && i == 0) {
+                        dq = l == 255 ? 1024 : l * 8;
+                    } else {
+                        int x = l * qmul;
+                        if (x > 0)
+                            x += qadd;
+                        else if (x < 0)
+                            x -= qadd;
+                        dq = clampi(x, -2048, 2047);
+                    }
+                    /* IDCT prescale (mbgpu:836-849) */
+                    const float cs = (float)((double)kInvPre[r] * 0.125);
+                    tv[r] = ((float)dq * pre_c) * cs;
+                }
+                /* IDCT column pass (mbgpu:851-888) */
+                idct_core(tv, o8);
+                __builtin_amdgcn_s_barrier(); /* all phase-2 reads of tf are done */
+#pragma unroll
+                for (int r = 0; r < 8; r++) tf[k8 * kBlkPitch + r * 8 + c] = o8[r];
+            } else {
+                __builtin_amdgcn_s_barrier();
+            }
+            __syncthreads();
+
+            /* ---- phase 3: lane (block, row): IDCT rows, reconstruction ---- */
+            if (L.act) {
+                float v[8], o8[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) v[k] = tf[k8 * kBlkPitch + rc * 8 + k];
+                idct_core(v, o8);
+                uint32_t w[2] = {0, 0};
+#pragma unroll
+                for (int x = 0; x < 8; x++) {
+                    const int rres = clampi((int)floor((double)o8[x] + 0.49999), -256, 255);
+                    const int pv = (int)((predw[x >> 2] >> (8 * (x & 3))) & 0xff);
+                    const int val = clampi(pv + rres, 0, 255);
+                    w[x >> 2] |= (uint32_t)val << (8 * (x & 3));
+                }
+                const StreamDesc &d = p.streams[L.s];
+                *(uint2 *)(d.rec[L.plane] + (size_t)(L.py + rc) * L.stride + L.px) = make_uint2(w[0], w[1]);
+            }
+            __syncthreads(); /* tf is reused by the next pass */
+        }
+
+        /* ---- per MB of the group: CBP, bit count of the CBP + TCOEFF
+         * layer, outputs ---- */
+#pragma unroll 1
+        for (int mi = 0; mi < G; mi++) {
+            const int g = g0 + mi;
+            if (g >= p.n_mb) break;
+            const int s = p.mb_stream[g];
+            if (s < 0) continue;
+            const StreamDesc &d = p.streams[s];
+            if (!(d.flags & kStreamActive)) continue;
+            const MbMode md = p.mode[g];
+            const int m = g - d.mb_base;
+            const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[s].pic_quant : d.pic_quant;
+            const int quant = clampi(picq + md.quant_adjust, 1, 31);
+            const bool intra = md.type == H261_MB_TYPE_INTRA;
+            int cbp = 0;
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+                if ((coded_bits >> (6 * mi + k)) & 1) cbp |= kCbpBit[k];
+            const int16_t *mlv = lv + 6 * mi * 64;
+
+            int bits = 0;
+            if (p.side && (intra || cbp)) {
+                /* lane k = transmission position k of each block in turn */
+                for (int k = 0; k < 6; k++) {
+                    const bool blk_written = intra || (cbp & kCbpBit[k]);
+                    if (!blk_written) continue;
+                    const int l = mlv[k * 64 + zz];
+                    const uint64_t nz = __ballot(l != 0);
+                    if (l != 0) {
+                        const uint64_t below = lane ? (nz & ((~0ull) >> (64 - lane))) : 0ull;
+                        const int prev = below ? 63 - __builtin_clzll(below) : -1;
+                        const int run = lane - prev - 1;
+                        if (lane == 0) {
+                            if (intra)
+                                bits += 8; /* DC: 8-bit fixed length */
+                            else if (l == 1 || l == -1)
+                                bits += 2; /* first coefficient '1s' */
+                            else
+                                bits += tcoeff_bits(tlen, 0, l);
+                        } else {
+                            bits += tcoeff_bits(tlen, run, l);
+                        }
+                    }
+                    if (lane == 0) bits += 2; /* EOB */
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) bits += __shfl_xor(bits, off, 64);
+                if (!intra) bits += c_cbp[cbp].len;
+            }
+
+            const int flags = md.flags | (cbp ? H261_MB_COEFF : 0);
+            if (p.levels) {
+                /* 384 levels as 96 dwords of 4 bytes: AC levels are within
+                 * [-127, 127], an intra DC within [1, 255] (read back as uint8) */
+                uint32_t *out = (uint32_t *)(p.levels + (size_t)g * 384);
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const int w = lane + 64 * j;
+                    if (w < 96) {
+                        const uint2 v = *(const uint2 *)&mlv[4 * w];
+                        out[w] = (v.x & 0xffu) | ((v.x >> 8) & 0xff00u) | ((v.y & 0xffu) << 16) | ((v.y >> 16) << 24);
+                    }
+                }
+            }
+            if (p.mb_struct) {
+                h261_macroblock_t *o = p.mb_struct + g;
+#pragma unroll
+                for (int k = 0; k < 6; k++) o->rgi_tcoeff[k][lane] = mlv[k * 64 + lane];
+                if (lane == 0) {
+                    o->i_macroblock_address = 0;
+                    o->i_macroblock_type = md.type;
+                    o->i_macroblock_type_flags = flags;
+                    o->i_macroblock_quant = quant;
+                    o->rgi_mv[0] = md.mvx;
+                    o->rgi_mv[1] = md.mvy;
+                    o->i_coded_block_pattern = cbp;
+                    o->i_mb_x = (m % d.mb_w) * 16;
+                    o->i_mb_y = (m / d.mb_w) * 16;
+                }
+            }
+            if (p.side && lane == 0) {
+                MbSide sd;
+                sd.type = (int8_t)md.type;
+                sd.flags = (int8_t)flags;
+                sd.quant = (int8_t)quant;
+                sd.cbp = (int8_t)cbp;
+                sd.mvx = (int8_t)md.mvx;
+                sd.mvy = (int8_t)md.mvy;
+                sd.bits = (int16_t)bits;
+                p.side[g] = sd;
+            }
+            if (lane == 0) atomicAdd(&scount[s == s0 ? 0 : 1][intra ? 0 : 1], 6);
         }
         __syncthreads();
 
-        /* ---- outputs ---- */
-        const int flags = md.flags | (cbp ? H261_MB_COEFF : 0);
-        if (p.levels) {
-            /* 384 levels as 96 dwords of 4 bytes: AC levels are within
-             * [-127, 127], an intra DC within [1, 255] (read back as uint8) */
-            uint32_t *out = (uint32_t *)(p.levels + (size_t)g * 384);
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int w = lane + 64 * j;
-                if (w < 96) {
-                    const int4 v = *(const int4 *)&lv[4 * w];
-                    out[w] = (uint32_t)(v.x & 0xff) | ((uint32_t)(v.y & 0xff) << 8) | ((uint32_t)(v.z & 0xff) << 16) |
-                             ((uint32_t)(v.w & 0xff) << 24);
-                }
-            }
-        }
-        if (p.mb_struct) {
-            h261_macroblock_t *o = p.mb_struct + g;
-#pragma unroll
-            for (int k = 0; k < 6; k++) o->rgi_tcoeff[k][lane] = lv[k * 64 + lane];
-            if (lane == 0) {
-                o->i_macroblock_address = 0;
-                o->i_macroblock_type = md.type;
-                o->i_macroblock_type_flags = flags;
-                o->i_macroblock_quant = quant;
-                o->rgi_mv[0] = md.mvx;
-                o->rgi_mv[1] = md.mvy;
-                o->i_coded_block_pattern = cbp;
-                o->i_mb_x = bx;
-                o->i_mb_y = by;
-            }
-        }
-        if (p.side && lane == 0) {
-            MbSide sd;
-            sd.type = (int8_t)md.type;
-            sd.flags = (int8_t)flags;
-            sd.quant = (int8_t)quant;
-            sd.cbp = (int8_t)cbp;
-            sd.mvx = (int8_t)md.mvx;
-            sd.mvy = (int8_t)md.mvy;
-            sd.bits = (int16_t)bits;
-            p.side[g] = sd;
-        }
-        if (lane == 0) atomicAdd(&scount[t], 6);
+        /* statistics of the group's streams go out at the group's end.  A
+         * stream has at least 99 MBs, so a group of 4 MBs meets at most two */
+        flush_stats(p, s0, stat[0], scount[0], lane);
+        const uint64_t has1 = __ballot(s1 >= 0);
+        if (has1) flush_stats(p, __builtin_amdgcn_readlane(s1, (int)__builtin_ctzll(has1)), stat[1], scount[1], lane);
         __syncthreads();
-        /* the segment of stream s in this wave ends here when the wave's next
-         * MB belongs to another stream (or there is none) */
-        const int gn = g + 1;
-        const bool last = it + 1 == p.mbs_per_wave || gn >= p.n_mb || p.mb_stream[gn] != s;
-        if (last) flush_segment(p, s, stat, scount, lane);
+        stat[0][0][lane] = stat[0][1][lane] = stat[1][0][lane] = stat[1][1][lane] = 0;
+        if (lane < 4) scount[lane >> 1][lane & 1] = 0;
     }
-    (void)sblk;
 }
 
 }  // namespace
@@ -544,11 +596,13 @@ int launch_mb(const MbLaunch &p, void *stream, void *ev_start, void *ev_stop)
     MbLaunch q = p;
     q.mbs_per_wave = per;
     const int grid = launch_grid((p.n_mb + per - 1) / per, p.xcd_remap);
+    /* groups of 4 MBs fill all 64 lanes; otherwise one MB per pass */
+    auto k = per % 4 == 0 ? mb_kernel<4> : mb_kernel<1>;
     if (ev_start)
-        hipExtLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, (hipEvent_t)ev_start,
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(64), 0, (hipStream_t)stream, (hipEvent_t)ev_start,
                               (hipEvent_t)ev_stop, 0, q);
     else
-        hipLaunchKernelGGL(mb_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, q);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, (hipStream_t)stream, q);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
