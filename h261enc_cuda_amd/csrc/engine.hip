@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <vector>
 
 #include "../../include/h261gpu.h"
@@ -55,11 +56,49 @@ struct Slot {
     uint8_t *input = nullptr;    /* host-fed frame buffer (I420) */
     uint8_t *recon = nullptr;    /* two I420 buffers */
     uint8_t *seq = nullptr;      /* device frame sequence */
+    size_t input_cap = 0, recon_cap = 0, seq_cap = 0; /* allocation sizes (device memory cache) */
     int seq_frames = 0;
     int64_t footprint = 0;
     bool pending = false;        /* host frame queued for the next step */
+    bool fresh = false;          /* no frame step since open: desc0/seq0 mirror the device */
+    StreamDesc desc0{};
+    SeqState seq0{};
     h261g_stream_params_t params{};
 };
+
+/* writes one stream's device state at open (kernel arguments carry it, so
+ * nothing waits for the host): descriptor, sequence state, denoise and rate
+ * control state, its MB range in the map, and a zeroed reconstruction chain
+ * (frame 0's reference is all zero, h261_main.c:26-28) */
+__global__ void stream_init_kernel(StreamDesc *desc, SeqState *seq, DenoiseState *dn, RcState *rc,
+                                   int32_t *mb_stream, int id, StreamDesc d, SeqState q, DenoiseState dnv,
+                                   RcState rcv, int base, int n_mb, uint32_t *recon, int recon_words)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, n = gridDim.x * blockDim.x;
+    for (int k = i; k < recon_words; k += n) recon[k] = 0;
+    for (int k = i; k < n_mb; k += n) mb_stream[base + k] = id;
+    if (i == 0) {
+        desc[id] = d;
+        seq[id] = q;
+        dn[id] = dnv;
+        rc[id] = rcv;
+    }
+}
+
+/* descriptor and sequence state of one stream (input selection) */
+__global__ void stream_set_kernel(StreamDesc *desc, SeqState *seq, int id, StreamDesc d, SeqState q)
+{
+    desc[id] = d;
+    seq[id] = q;
+}
+
+/* a closed stream: inactive descriptor, MB range unmapped */
+__global__ void stream_close_kernel(StreamDesc *desc, int32_t *mb_stream, int id, int base, int n_mb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_mb) mb_stream[base + i] = -1;
+    if (i == 0) desc[id] = StreamDesc{};
+}
 
 }  // namespace
 
@@ -70,6 +109,11 @@ using namespace h261g;
 struct h261g_engine {
     int max_streams = 0;
     int64_t budget = 0, used = 0;
+    /* device memory cache of closed streams' buffers (size -> block): open
+     * and close neither allocate nor free nor wait when a block fits.  All
+     * work is on one stream, so a block reused by a later open is only
+     * touched after the work that used it before. */
+    std::multimap<size_t, uint8_t *> cache;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_variant = kMeTwoStage;
     int fused_tails = 1;
@@ -167,6 +211,40 @@ int ensure_mb_capacity(h261g_engine *e, int n)
 void invalidate_graph(h261g_engine *e)
 {
     e->graph_valid = false;
+}
+
+void cache_release(h261g_engine *e)
+{
+    if (e->cache.empty()) return;
+    (void)hipStreamSynchronize(e->stream);
+    for (auto &kv : e->cache) hipFree(kv.second);
+    e->cache.clear();
+}
+
+/* a block of at least `bytes` (and at most twice that) from the cache, else
+ * a new allocation; *cap gets the block's size */
+uint8_t *cache_get(h261g_engine *e, size_t bytes, size_t *cap)
+{
+    auto it = e->cache.lower_bound(bytes);
+    if (it != e->cache.end() && it->first <= 2 * bytes) {
+        uint8_t *ptr = it->second;
+        *cap = it->first;
+        e->cache.erase(it);
+        return ptr;
+    }
+    uint8_t *ptr = nullptr;
+    if (hipMalloc((void **)&ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        cache_release(e);
+        if (!ok(hipMalloc((void **)&ptr, bytes), "hipMalloc")) return nullptr;
+    }
+    *cap = bytes;
+    return ptr;
+}
+
+void cache_put(h261g_engine *e, uint8_t *ptr, size_t cap)
+{
+    if (ptr) e->cache.emplace(cap, ptr);
 }
 
 int stream_count(const h261g_engine *e) { return e->max_streams; }
@@ -357,6 +435,8 @@ int h261g_engine_destroy(h261g_engine_t *e)
         hipFree(s.recon);
         hipFree(s.seq);
     }
+    for (auto &kv : e->cache) hipFree(kv.second);
+    e->cache.clear();
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     hipFree(e->desc);
@@ -400,9 +480,10 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
     s.mb_h = p->height / 16;
     s.n_mb = s.mb_w * s.mb_h;
     s.params = *p;
-    if (!ok(hipMalloc((void **)&s.input, fsz), "hipMalloc") || !ok(hipMalloc((void **)&s.recon, 2 * fsz), "hipMalloc")) {
-        hipFree(s.input);
-        hipFree(s.recon);
+    s.input = cache_get(e, fsz, &s.input_cap);
+    s.recon = s.input ? cache_get(e, 2 * fsz, &s.recon_cap) : nullptr;
+    if (!s.input || !s.recon) {
+        cache_put(e, s.input, s.input_cap);
         s = Slot();
         return H261G_ERR_MEMORY;
     }
@@ -416,8 +497,8 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
         base = e->mb_used;
         int rcode = ensure_mb_capacity(e, e->mb_used + s.n_mb);
         if (rcode) {
-            hipFree(s.input);
-            hipFree(s.recon);
+            cache_put(e, s.input, s.input_cap);
+            cache_put(e, s.recon, s.recon_cap);
             s = Slot();
             return rcode;
         }
@@ -467,16 +548,12 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
     rc.vbv_size = p->vbv_size / 8;
     rc.vbv_fill = rc.vbv_size;
     rc.fixed_quant = p->fixed_quant;
-    bool good = ok(hipMemsetAsync(s.recon, 0, 2 * fsz, e->stream), "hipMemset") &&
-                ok(hipMemcpyAsync(e->mb_stream + base, map.data(), sizeof(int32_t) * s.n_mb, hipMemcpyHostToDevice,
-                                  e->stream),
-                   "hipMemcpy") &&
-                ok(hipMemcpyAsync(e->desc + id, &d, sizeof(d), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
-                ok(hipMemcpyAsync(e->seq + id, &q, sizeof(q), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
-                ok(hipMemcpyAsync(e->dn + id, &dn, sizeof(dn), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
-                ok(hipMemcpyAsync(e->rc + id, &rc, sizeof(rc), hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
-                ok(hipStreamSynchronize(e->stream), "sync");
-    if (!good) return H261G_ERR_DEVICE;
+    hipLaunchKernelGGL(stream_init_kernel, dim3(64), dim3(256), 0, e->stream, e->desc, e->seq, e->dn, e->rc,
+                       e->mb_stream, id, d, q, dn, rc, base, s.n_mb, (uint32_t *)s.recon, (int)(2 * fsz / 4));
+    if (!ok(hipGetLastError(), "stream init")) return H261G_ERR_DEVICE;
+    s.desc0 = d;
+    s.seq0 = q;
+    s.fresh = true;
     invalidate_graph(e);
     return id;
 }
@@ -485,15 +562,15 @@ int h261g_stream_close(h261g_engine_t *e, int32_t id)
 {
     Slot *s = get_slot(e, id);
     if (!s) return H261G_ERR_ARG;
-    if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
-    StreamDesc d{};
-    if (!ok(hipMemcpy(e->desc + id, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy")) return H261G_ERR_DEVICE;
-    hipFree(s->input);
-    hipFree(s->recon);
-    hipFree(s->seq);
+    /* ordered after every step that used the stream; the buffers go to the
+     * cache (a later open reuses them after this point on the same stream) */
+    hipLaunchKernelGGL(stream_close_kernel, dim3((s->n_mb + 255) / 256), dim3(256), 0, e->stream, e->desc, e->mb_stream,
+                       id, s->mb_base, s->n_mb);
+    if (!ok(hipGetLastError(), "stream close")) return H261G_ERR_DEVICE;
+    cache_put(e, s->input, s->input_cap);
+    cache_put(e, s->recon, s->recon_cap);
+    cache_put(e, s->seq, s->seq_cap);
     e->used -= s->footprint;
-    /* unmap the range (kernels skip MBs mapped to -1) and keep it for reuse */
-    if (!ok(hipMemset(e->mb_stream + s->mb_base, 0xff, sizeof(int32_t) * s->n_mb), "hipMemset")) return H261G_ERR_DEVICE;
     (s->n_mb == 396 ? e->free_cif : e->free_qcif).push_back(s->mb_base);
     *s = Slot();
     invalidate_graph(e);
@@ -562,32 +639,42 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     const int64_t fsz = (int64_t)s->width * s->height * 3 / 2;
     const int64_t bytes = fsz * n_frames;
     if (e->budget && e->used + bytes > e->budget) return H261G_ERR_FULL;
-    if (!ok(hipMalloc((void **)&s->seq, bytes), "hipMalloc")) return H261G_ERR_MEMORY;
+    s->seq = cache_get(e, (size_t)bytes, &s->seq_cap);
+    if (!s->seq) return H261G_ERR_MEMORY;
     int rcode = generate_frames_device(*c, s->width, s->height, 0, n_frames, s->seq, e->stream);
     if (rcode) {
-        hipFree(s->seq);
+        cache_put(e, s->seq, s->seq_cap);
         s->seq = nullptr;
         return rcode;
     }
     e->used += bytes;
     s->footprint += bytes;
     s->seq_frames = n_frames;
-    /* point the descriptor at frame 0 and activate */
+    /* point the descriptor at the next frame to code and activate: from the
+     * host mirror while no step has run since open, else read back */
     StreamDesc d;
-    if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
+    SeqState q;
+    if (s->fresh) {
+        d = s->desc0;
+        q = s->seq0;
+    } else if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy") ||
+               !ok(hipMemcpy(&q, e->seq + id, sizeof(q), hipMemcpyDeviceToHost), "hipMemcpy")) {
+        return H261G_ERR_DEVICE;
+    }
     const size_t ysz = (size_t)s->width * s->height, csz = ysz / 4;
     const uint8_t *cur = s->seq + (size_t)fsz * d.frame_num;
     d.cur[0] = cur;
     d.cur[1] = cur + ysz;
     d.cur[2] = cur + ysz + csz;
     d.flags = kStreamActive | kStreamDeviceQuant | kStreamSeqInput;
-    SeqState q;
-    if (!ok(hipMemcpy(&q, e->seq + id, sizeof(q), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
     q.base = s->seq;
     q.remaining = n_frames - d.frame_num;
-    if (!ok(hipMemcpy(e->desc + id, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy") ||
-        !ok(hipMemcpy(e->seq + id, &q, sizeof(q), hipMemcpyHostToDevice), "hipMemcpy"))
-        return H261G_ERR_DEVICE;
+    hipLaunchKernelGGL(stream_set_kernel, dim3(1), dim3(1), 0, e->stream, e->desc, e->seq, id, d, q);
+    if (!ok(hipGetLastError(), "stream set")) return H261G_ERR_DEVICE;
+    if (s->fresh) {
+        s->desc0 = d;
+        s->seq0 = q;
+    }
     return H261G_OK;
 }
 
@@ -599,6 +686,7 @@ int h261g_engine_step(h261g_engine_t *e)
         if (s.open && (s.pending || s.seq)) n++;
     int rcode = enqueue_step(e, e->profile);
     if (rcode) return rcode;
+    for (auto &s : e->slots) s.fresh = false;
 
     for (auto &s : e->slots) s.pending = false;
     return n;
@@ -629,6 +717,8 @@ int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
     }
     for (int i = 0; i < n_steps; i++)
         if (!ok(hipGraphLaunch(e->graph_exec, e->stream), "graph launch")) return H261G_ERR_DEVICE;
+    if (n_steps > 0)
+        for (auto &s : e->slots) s.fresh = false;
     for (auto &s : e->slots) s.pending = false;
     return H261G_OK;
 }

@@ -87,20 +87,24 @@ class ServingLoop:
         eng = self.eng
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting: collections.deque = collections.deque()
-        active: Dict[int, dict] = {}
-        frame_lat: List[float] = []
-        req_lat: List[float] = []
-        outputs: Dict[int, List[bytes]] = {}
         index = {id(r): i for i, r in enumerate(requests)}
+        outputs: Dict[int, List[bytes]] = {}
+        # active streams as parallel arrays (per-step bookkeeping is vectorised)
+        sids = np.zeros(0, np.int64)
+        req_idx = np.zeros(0, np.int64)
+        eligible = np.zeros(0)
+        arrival = np.zeros(0)
+        left = np.zeros(0, np.int64)
+        nmb = np.zeros(0, np.int64)
+        frame_lat: List[np.ndarray] = []
+        req_lat: List[float] = []
         t_start = time.perf_counter()
-        frames = 0
-        mbs = 0
-        steps = 0
-        max_conc = 0
-        while pending or waiting or active:
+        frames = mbs = steps = max_conc = 0
+        while pending or waiting or len(sids):
             now = time.perf_counter() - t_start
             while pending and (pending[0].arrival <= now or not realtime):
                 waiting.append(pending.popleft())
+            new = []
             while waiting:
                 r = waiting[0]
                 try:
@@ -115,37 +119,44 @@ class ServingLoop:
                     eng.close(sid)
                     break
                 waiting.popleft()
-                t_adm = time.perf_counter() - t_start
-                active[sid] = dict(req=r, done=0, eligible=t_adm, arrival=r.arrival if realtime else 0.0)
+                new.append((sid, r))
                 if collect:
                     outputs[index[id(r)]] = []
-            if not active:
+            if new:
+                t_adm = time.perf_counter() - t_start
+                sids = np.concatenate([sids, [sid for sid, _ in new]])
+                req_idx = np.concatenate([req_idx, [index[id(r)] for _, r in new]])
+                eligible = np.concatenate([eligible, np.full(len(new), t_adm)])
+                arrival = np.concatenate([arrival, [r.arrival if realtime else 0.0 for _, r in new]])
+                left = np.concatenate([left, [r.frames for _, r in new]])
+                nmb = np.concatenate([nmb, [(r.width // 16) * (r.height // 16) for _, r in new]])
+            if not len(sids):
                 if pending and realtime:
                     time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t_start), 0.01)))
                 continue
-            max_conc = max(max_conc, len(active))
+            max_conc = max(max_conc, len(sids))
             eng.step()
             eng.sync()
             steps += 1
             t_end = time.perf_counter() - t_start
-            finished = []
-            for sid, a in active.items():
-                r = a["req"]
-                frame_lat.append((t_end - a["eligible"]) * 1e3)
-                a["eligible"] = t_end
-                a["done"] += 1
-                frames += 1
-                mbs += (r.width // 16) * (r.height // 16)
-                if collect:
-                    outputs[index[id(r)]].append(eng.bitstream(sid))
-                if a["done"] >= r.frames:
-                    finished.append(sid)
-                    req_lat.append((t_end - a["arrival"]) * 1e3)
-            for sid in finished:
-                eng.close(sid)
-                del active[sid]
+            frame_lat.append((t_end - eligible) * 1e3)
+            eligible[:] = t_end
+            left -= 1
+            frames += len(sids)
+            mbs += int(nmb.sum())
+            if collect:
+                for sid, ri in zip(sids.tolist(), req_idx.tolist()):
+                    outputs[ri].append(eng.bitstream(sid))
+            done = left <= 0
+            if done.any():
+                for sid in sids[done].tolist():
+                    eng.close(sid)
+                req_lat.extend(((t_end - arrival[done]) * 1e3).tolist())
+                keep = ~done
+                sids, req_idx, eligible, arrival, left, nmb = (sids[keep], req_idx[keep], eligible[keep],
+                                                                arrival[keep], left[keep], nmb[keep])
         seconds = time.perf_counter() - t_start
-        fl = np.array(frame_lat) if frame_lat else np.zeros(1)
+        fl = np.concatenate(frame_lat) if frame_lat else np.zeros(1)
         rl = np.array(req_lat) if req_lat else np.zeros(1)
         stats = ServingStats(requests=len(requests), frames=frames, cif_equivalent_frames=mbs / 396.0,
                              seconds=seconds, frame_latency_ms_p50=float(np.percentile(fl, 50)),
