@@ -236,6 +236,7 @@ def serving(rate=2000.0, duration=0.5):
             "frame_latency_ms_p99": round(st.frame_latency_ms_p99, 3),
             "request_latency_ms_p50": round(st.request_latency_ms_p50, 1),
             "request_latency_ms_p99": round(st.request_latency_ms_p99, 1),
+            "input_wait_ms_p50": round(st.input_wait_ms_p50, 2), "input_wait_ms_p99": round(st.input_wait_ms_p99, 2),
             "max_concurrent_streams": st.max_concurrent, "steps": st.steps}
 
 

@@ -50,12 +50,13 @@ EXPORTS = [
     "h261g_engine_sync", "h261g_stream_info", "h261g_stream_read_macroblocks", "h261g_stream_read_recon",
     "h261g_stream_read_me", "h261g_stream_read_bitstream", "h261g_stream_read_input",
     "h261g_engine_profile", "h261g_engine_kernel_times", "h261g_content_frame_host", "h261g_write_picture",
-    "h261g_engine_config", "h261g_device_info",
+    "h261g_engine_config", "h261g_device_info", "h261g_stream_input_pending",
 ]
 OPT_ME_VARIANT = 1
 OPT_MBS_PER_WAVE = 2
 OPT_FUSED_TAILS = 3
 OPT_XCD_REMAP = 4
+OPT_ASYNC_INPUT = 5
 
 
 class StreamParams(ctypes.Structure):
@@ -107,6 +108,7 @@ def load() -> ctypes.CDLL:
         "h261g_stream_put_frame": (ctypes.c_int, [P, i32, P, P, P]),
         "h261g_stream_generate_input": (ctypes.c_int, [P, i32, ctypes.POINTER(ContentParams), i32]),
         "h261g_stream_frames_done": (ctypes.c_int, [P, i32]),
+        "h261g_stream_input_pending": (ctypes.c_int, [P, i32]),
         "h261g_engine_compact": (ctypes.c_int, [P]),
         "h261g_engine_step": (ctypes.c_int, [P]),
         "h261g_engine_run": (ctypes.c_int, [P, i32]),

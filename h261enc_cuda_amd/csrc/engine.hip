@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <vector>
 
@@ -63,6 +64,12 @@ struct Slot {
     bool fresh = false;          /* no frame step since open: desc0/seq0 mirror the device */
     StreamDesc desc0{};
     SeqState seq0{};
+    /* asynchronous input: generation on the input stream, activation by the
+     * first step after gen_ev completes */
+    bool gen_pending = false;
+    hipEvent_t gen_ev = nullptr;
+    StreamDesc gen_d{};
+    SeqState gen_q{};
     h261g_stream_params_t params{};
 };
 
@@ -92,6 +99,17 @@ __global__ void stream_set_kernel(StreamDesc *desc, SeqState *seq, int id, Strea
     seq[id] = q;
 }
 
+/* re-packing of the MB index space: the new map, and each open stream's new
+ * MB base (bases[id] < 0: slot not open) */
+__global__ void remap_kernel(StreamDesc *desc, int32_t *mb_stream, const int32_t *bases, const int32_t *map, int n,
+                             int max_streams)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int k = i; k < n; k += stride) mb_stream[k] = map[k];
+    for (int k = i; k < max_streams; k += stride)
+        if (bases[k] >= 0) desc[k].mb_base = bases[k];
+}
+
 /* a closed stream: inactive descriptor, MB range unmapped */
 __global__ void stream_close_kernel(StreamDesc *desc, int32_t *mb_stream, int id, int base, int n_mb)
 {
@@ -114,6 +132,17 @@ struct h261g_engine {
      * work is on one stream, so a block reused by a later open is only
      * touched after the work that used it before. */
     std::multimap<size_t, uint8_t *> cache;
+    /* re-packing: pinned host staging (bases then map), its device copy, and
+     * the event after which the host staging may be rewritten */
+    int32_t *remap_host = nullptr, *remap_dev = nullptr;
+    size_t remap_cap = 0;
+    hipEvent_t remap_ev = nullptr;
+    bool remap_pending = false;
+    /* input generation stream (default priority, like the compute stream) and
+     * the event marking the compute work a generation must follow */
+    hipStream_t in_stream = nullptr;
+    hipEvent_t compute_ev = nullptr;
+    int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_variant = kMeTwoStage;
     int fused_tails = 1;
@@ -191,8 +220,15 @@ int ensure_mb_capacity(h261g_engine *e, int n)
         hipFree(lv);
         return H261G_ERR_MEMORY;
     }
-    if (e->mb_stream && e->mb_used)
-        (void)hipMemcpy(ms, e->mb_stream, sizeof(int32_t) * e->mb_used, hipMemcpyDeviceToDevice);
+    /* the map and the last step's results of the MBs in use survive growth */
+    if (e->mb_stream && e->mb_used) {
+        const size_t n = (size_t)e->mb_used;
+        (void)hipMemcpy(ms, e->mb_stream, sizeof(int32_t) * n, hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(me, e->me, sizeof(MeResult) * n, hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(mo, e->mode, sizeof(MbMode) * n, hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(sd, e->side, sizeof(MbSide) * n, hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(lv, e->levels, (size_t)384 * n, hipMemcpyDeviceToDevice);
+    }
     hipFree(e->mb_stream);
     hipFree(e->me);
     hipFree(e->mode);
@@ -368,6 +404,10 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         if (value < 0 || value > 3) return H261G_ERR_ARG;
         e->xcd_remap = value;
         break;
+    case H261G_OPT_ASYNC_INPUT:
+        if (value != 0 && value != 1) return H261G_ERR_ARG;
+        e->async_input = value;
+        return H261G_OK;
     default:
         return H261G_ERR_ARG;
     }
@@ -410,6 +450,8 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
     e->mbs_per_wave = mbs_per_wave;
     e->slots.resize(max_streams);
     bool good = ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "stream") &&
+                ok(hipStreamCreateWithFlags(&e->in_stream, hipStreamNonBlocking), "stream") &&
+                ok(hipEventCreateWithFlags(&e->compute_ev, hipEventDisableTiming), "event") &&
                 ok(hipMalloc((void **)&e->desc, sizeof(StreamDesc) * max_streams), "hipMalloc") &&
                 ok(hipMalloc((void **)&e->seq, sizeof(SeqState) * max_streams), "hipMalloc") &&
                 ok(hipMalloc((void **)&e->dn, sizeof(DenoiseState) * max_streams), "hipMalloc") &&
@@ -430,13 +472,18 @@ int h261g_engine_destroy(h261g_engine_t *e)
 {
     if (!e) return H261G_ERR_ARG;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->in_stream) (void)hipStreamSynchronize(e->in_stream);
     for (auto &s : e->slots) {
+        if (s.gen_ev) (void)hipEventDestroy(s.gen_ev);
         hipFree(s.input);
         hipFree(s.recon);
         hipFree(s.seq);
     }
     for (auto &kv : e->cache) hipFree(kv.second);
     e->cache.clear();
+    hipHostFree(e->remap_host);
+    hipFree(e->remap_dev);
+    if (e->remap_ev) (void)hipEventDestroy(e->remap_ev);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     hipFree(e->desc);
@@ -452,6 +499,8 @@ int h261g_engine_destroy(h261g_engine_t *e)
     for (int r = 0; r < h261g_engine::kEvRing; r++)
         for (int i = 0; i < 6; i++)
             if (e->ev[r][i]) (void)hipEventDestroy(e->ev[r][i]);
+    if (e->compute_ev) (void)hipEventDestroy(e->compute_ev);
+    if (e->in_stream) (void)hipStreamDestroy(e->in_stream);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return H261G_OK;
@@ -473,7 +522,9 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
     if (e->budget && e->used + fp > e->budget) return H261G_ERR_FULL;
     Slot &s = e->slots[id];
     const int64_t fsz = (int64_t)p->width * p->height * 3 / 2;
+    hipEvent_t gen_ev = s.gen_ev; /* a slot keeps its event object across streams */
     s = Slot();
+    s.gen_ev = gen_ev;
     s.width = p->width;
     s.height = p->height;
     s.mb_w = p->width / 16;
@@ -485,6 +536,7 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
     if (!s.input || !s.recon) {
         cache_put(e, s.input, s.input_cap);
         s = Slot();
+        s.gen_ev = gen_ev;
         return H261G_ERR_MEMORY;
     }
     /* MB range: a free range of a closed stream of the same size, else append */
@@ -500,6 +552,7 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
             cache_put(e, s.input, s.input_cap);
             cache_put(e, s.recon, s.recon_cap);
             s = Slot();
+            s.gen_ev = gen_ev;
             return rcode;
         }
         e->mb_used += s.n_mb;
@@ -562,6 +615,9 @@ int h261g_stream_close(h261g_engine_t *e, int32_t id)
 {
     Slot *s = get_slot(e, id);
     if (!s) return H261G_ERR_ARG;
+    /* an input still being generated finishes before its buffer is released */
+    if (s->gen_pending && !ok(hipStreamWaitEvent(e->stream, s->gen_ev, 0), "wait")) return H261G_ERR_DEVICE;
+    hipEvent_t gen_ev = s->gen_ev;
     /* ordered after every step that used the stream; the buffers go to the
      * cache (a later open reuses them after this point on the same stream) */
     hipLaunchKernelGGL(stream_close_kernel, dim3((s->n_mb + 255) / 256), dim3(256), 0, e->stream, e->desc, e->mb_stream,
@@ -573,6 +629,7 @@ int h261g_stream_close(h261g_engine_t *e, int32_t id)
     e->used -= s->footprint;
     (s->n_mb == 396 ? e->free_cif : e->free_qcif).push_back(s->mb_base);
     *s = Slot();
+    s->gen_ev = gen_ev; /* the slot keeps its event for the next stream */
     invalidate_graph(e);
     return H261G_OK;
 }
@@ -593,6 +650,13 @@ int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_t *y, cons
     return good ? H261G_OK : H261G_ERR_DEVICE;
 }
 
+int h261g_stream_input_pending(h261g_engine_t *e, int32_t id)
+{
+    Slot *s = get_slot(e, id);
+    if (!s) return H261G_ERR_ARG;
+    return s->gen_pending ? 1 : 0;
+}
+
 int h261g_stream_frames_done(h261g_engine_t *e, int32_t id)
 {
     Slot *s = get_slot(e, id);
@@ -606,30 +670,59 @@ int h261g_stream_frames_done(h261g_engine_t *e, int32_t id)
 
 /* per-MB arrays only carry data within a step, so re-packing needs only the
  * map and each descriptor's mb_base rewritten */
+/* per-MB arrays only carry data within a step, so re-packing needs only the
+ * map and each descriptor's mb_base rewritten.  The host knows both, so the
+ * update is queued (pinned staging copy + a kernel) and nothing waits */
 int h261g_engine_compact(h261g_engine_t *e)
 {
     if (!e) return H261G_ERR_ARG;
-    if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
-    std::vector<StreamDesc> d(e->max_streams);
-    if (!ok(hipMemcpy(d.data(), e->desc, sizeof(StreamDesc) * e->max_streams, hipMemcpyDeviceToHost), "hipMemcpy"))
+    int n = 0;
+    for (int i = 0; i < e->max_streams; i++)
+        if (e->slots[i].open) n += e->slots[i].n_mb;
+    const size_t words = (size_t)e->max_streams + n;
+    if (e->remap_pending && !ok(hipEventSynchronize(e->remap_ev), "event")) return H261G_ERR_DEVICE;
+    e->remap_pending = false;
+    if (words > e->remap_cap) {
+        if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
+        hipHostFree(e->remap_host);
+        hipFree(e->remap_dev);
+        e->remap_host = nullptr;
+        e->remap_dev = nullptr;
+        e->remap_cap = 0;
+        const size_t cap = words + words / 2;
+        if (!ok(hipHostMalloc((void **)&e->remap_host, cap * sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc") ||
+            !ok(hipMalloc((void **)&e->remap_dev, cap * sizeof(int32_t)), "hipMalloc"))
+            return H261G_ERR_MEMORY;
+        e->remap_cap = cap;
+    }
+    if (!e->remap_ev && !ok(hipEventCreateWithFlags(&e->remap_ev, hipEventDisableTiming), "event"))
         return H261G_ERR_DEVICE;
-    std::vector<int32_t> map;
+    int32_t *bases = e->remap_host, *map = e->remap_host + e->max_streams;
+    int used = 0;
     for (int i = 0; i < e->max_streams; i++) {
         Slot &s = e->slots[i];
+        bases[i] = -1;
         if (!s.open) continue;
-        s.mb_base = (int)map.size();
-        d[i].mb_base = s.mb_base;
-        map.insert(map.end(), s.n_mb, i);
+        s.mb_base = used;
+        s.desc0.mb_base = used;
+        bases[i] = used;
+        for (int k = 0; k < s.n_mb; k++) map[used + k] = i;
+        used += s.n_mb;
     }
-    e->mb_used = (int)map.size();
+    e->mb_used = used;
     e->free_cif.clear();
     e->free_qcif.clear();
-    bool good = (map.empty() || ok(hipMemcpy(e->mb_stream, map.data(), sizeof(int32_t) * map.size(),
-                                             hipMemcpyHostToDevice), "hipMemcpy")) &&
-                ok(hipMemcpy(e->desc, d.data(), sizeof(StreamDesc) * e->max_streams, hipMemcpyHostToDevice),
-                   "hipMemcpy");
+    if (!ok(hipMemcpyAsync(e->remap_dev, e->remap_host, words * sizeof(int32_t), hipMemcpyHostToDevice, e->stream),
+            "hipMemcpy") ||
+        !ok(hipEventRecord(e->remap_ev, e->stream), "event"))
+        return H261G_ERR_DEVICE;
+    e->remap_pending = true;
+    const int threads = 256, blocks = (int)std::min<size_t>(1024, (words + threads - 1) / threads);
+    hipLaunchKernelGGL(remap_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, e->stream, e->desc, e->mb_stream,
+                       e->remap_dev, e->remap_dev + e->max_streams, used, e->max_streams);
+    if (!ok(hipGetLastError(), "remap")) return H261G_ERR_DEVICE;
     invalidate_graph(e);
-    return good ? H261G_OK : H261G_ERR_DEVICE;
+    return H261G_OK;
 }
 
 int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_content_params_t *c, int32_t n_frames)
@@ -641,7 +734,16 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     if (e->budget && e->used + bytes > e->budget) return H261G_ERR_FULL;
     s->seq = cache_get(e, (size_t)bytes, &s->seq_cap);
     if (!s->seq) return H261G_ERR_MEMORY;
-    int rcode = generate_frames_device(*c, s->width, s->height, 0, n_frames, s->seq, e->stream);
+    /* asynchronous input (a fresh stream only): generated on the input
+     * stream after the compute work queued so far (a cached buffer may still
+     * be read by it); the stream joins the first step after it is ready */
+    const bool async = e->async_input && s->fresh;
+    if (async && !s->gen_ev && !ok(hipEventCreateWithFlags(&s->gen_ev, hipEventDisableTiming), "event"))
+        return H261G_ERR_DEVICE;
+    if (async && (!ok(hipEventRecord(e->compute_ev, e->stream), "event") ||
+                  !ok(hipStreamWaitEvent(e->in_stream, e->compute_ev, 0), "wait")))
+        return H261G_ERR_DEVICE;
+    int rcode = generate_frames_device(*c, s->width, s->height, 0, n_frames, s->seq, async ? e->in_stream : e->stream);
     if (rcode) {
         cache_put(e, s->seq, s->seq_cap);
         s->seq = nullptr;
@@ -669,6 +771,15 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     d.flags = kStreamActive | kStreamDeviceQuant | kStreamSeqInput;
     q.base = s->seq;
     q.remaining = n_frames - d.frame_num;
+    if (async) {
+        if (!ok(hipEventRecord(s->gen_ev, e->in_stream), "event")) return H261G_ERR_DEVICE;
+        s->gen_pending = true;
+        s->gen_d = d;
+        s->gen_q = q;
+        s->desc0 = d;
+        s->seq0 = q;
+        return H261G_OK;
+    }
     hipLaunchKernelGGL(stream_set_kernel, dim3(1), dim3(1), 0, e->stream, e->desc, e->seq, id, d, q);
     if (!ok(hipGetLastError(), "stream set")) return H261G_ERR_DEVICE;
     if (s->fresh) {
@@ -678,9 +789,29 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     return H261G_OK;
 }
 
+/* streams whose asynchronous input is ready join the next step */
+int activate_ready_inputs(h261g_engine *e)
+{
+    for (int id = 0; id < e->max_streams; id++) {
+        Slot &s = e->slots[id];
+        if (!s.open || !s.gen_pending) continue;
+        const hipError_t q = hipEventQuery(s.gen_ev);
+        if (q == hipErrorNotReady) continue;
+        if (!ok(q, "event")) return H261G_ERR_DEVICE;
+        hipLaunchKernelGGL(stream_set_kernel, dim3(1), dim3(1), 0, e->stream, e->desc, e->seq, id, s.gen_d, s.gen_q);
+        if (!ok(hipGetLastError(), "stream set")) return H261G_ERR_DEVICE;
+        s.gen_pending = false;
+    }
+    return H261G_OK;
+}
+
 int h261g_engine_step(h261g_engine_t *e)
 {
     if (!e) return H261G_ERR_ARG;
+    if (e->async_input) {
+        int rc0 = activate_ready_inputs(e);
+        if (rc0) return rc0;
+    }
     int n = 0;
     for (auto &s : e->slots)
         if (s.open && (s.pending || s.seq)) n++;
@@ -695,6 +826,10 @@ int h261g_engine_step(h261g_engine_t *e)
 int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
 {
     if (!e || n_steps < 0) return H261G_ERR_ARG;
+    if (e->async_input) {
+        int rc0 = activate_ready_inputs(e);
+        if (rc0) return rc0;
+    }
     if (e->profile) {
         for (int i = 0; i < n_steps; i++) {
             int rcode = h261g_engine_step(e);

@@ -103,6 +103,9 @@ class Engine:
     def compact(self) -> None:
         check(self.lib.h261g_engine_compact(self.h), "compact")
 
+    def input_pending(self, sid: int) -> bool:
+        return bool(check(self.lib.h261g_stream_input_pending(self.h, sid), "input_pending"))
+
     def frames_done(self, sid: int) -> int:
         return check(self.lib.h261g_stream_frames_done(self.h, sid), "frames_done")
 

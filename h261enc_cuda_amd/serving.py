@@ -7,9 +7,12 @@ one frame step over all admitted streams -- requests that arrive while
 others are coding join the next step (continuous batching); finished
 streams are closed and their memory and macroblock range reused.
 
-Per-frame latency = time from the frame becoming eligible (its request
-admitted and its previous frame done) to the end of the step that coded it
-(levels, side information and the exact frame size on the device).
+Per-frame latency = time from the frame becoming eligible (its input
+available and its previous frame done) to the end of the step that coded it
+(levels, side information and the exact frame size on the device).  A
+request's input is a synthetic sequence generated on the device; the time
+from its arrival until that input is available is reported separately as
+the input wait, and is part of the request latency (arrival to last frame).
 """
 from __future__ import annotations
 
@@ -20,6 +23,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
+from . import _native as nat
 from .encoder import Engine
 
 
@@ -65,6 +69,8 @@ class ServingStats:
     frame_latency_ms_p99: float
     request_latency_ms_p50: float
     request_latency_ms_p99: float
+    input_wait_ms_p50: float
+    input_wait_ms_p99: float
     max_concurrent: int
     steps: int
 
@@ -78,8 +84,12 @@ class ServingStats:
 
 
 class ServingLoop:
-    def __init__(self, hbm_budget: int = 64 << 30, max_streams: int = 4096, mbs_per_wave: int = 0):
+    def __init__(self, hbm_budget: int = 64 << 30, max_streams: int = 4096, mbs_per_wave: int = 0,
+                 async_input: bool = True):
         self.eng = Engine(max_streams=max_streams, hbm_budget=hbm_budget, mbs_per_wave=mbs_per_wave)
+        # a request's frames are generated beside the frame steps; it joins
+        # the first step after they are ready
+        self.eng.config(nat.OPT_ASYNC_INPUT, int(async_input))
 
     def run(self, requests: List[Request], collect: bool = False, realtime: bool = True):
         """Serves the requests.  collect=True also returns every request's
@@ -96,8 +106,10 @@ class ServingLoop:
         arrival = np.zeros(0)
         left = np.zeros(0, np.int64)
         nmb = np.zeros(0, np.int64)
+        started = np.zeros(0, bool)  # input ready: the stream codes a frame per step
         frame_lat: List[np.ndarray] = []
         req_lat: List[float] = []
+        input_wait: List[float] = []
         t_start = time.perf_counter()
         frames = mbs = steps = max_conc = 0
         while pending or waiting or len(sids):
@@ -130,22 +142,32 @@ class ServingLoop:
                 arrival = np.concatenate([arrival, [r.arrival if realtime else 0.0 for _, r in new]])
                 left = np.concatenate([left, [r.frames for _, r in new]])
                 nmb = np.concatenate([nmb, [(r.width // 16) * (r.height // 16) for _, r in new]])
+                started = np.concatenate([started, np.zeros(len(new), bool)])
             if not len(sids):
                 if pending and realtime:
                     time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t_start), 0.01)))
                 continue
             max_conc = max(max_conc, len(sids))
+            t_call = time.perf_counter() - t_start
             eng.step()
+            # streams activated by this step (their input became ready) code
+            # their first frame in it: eligible from this step's start
+            for k in np.nonzero(~started)[0].tolist():
+                if not eng.input_pending(int(sids[k])):
+                    started[k] = True
+                    eligible[k] = t_call
+                    input_wait.append((t_call - arrival[k]) * 1e3)
             eng.sync()
             steps += 1
             t_end = time.perf_counter() - t_start
-            frame_lat.append((t_end - eligible) * 1e3)
-            eligible[:] = t_end
-            left -= 1
-            frames += len(sids)
-            mbs += int(nmb.sum())
+            run = started
+            frame_lat.append((t_end - eligible[run]) * 1e3)
+            eligible[run] = t_end
+            left[run] -= 1
+            frames += int(run.sum())
+            mbs += int(nmb[run].sum())
             if collect:
-                for sid, ri in zip(sids.tolist(), req_idx.tolist()):
+                for sid, ri in zip(sids[run].tolist(), req_idx[run].tolist()):
                     outputs[ri].append(eng.bitstream(sid))
             done = left <= 0
             if done.any():
@@ -153,16 +175,19 @@ class ServingLoop:
                     eng.close(sid)
                 req_lat.extend(((t_end - arrival[done]) * 1e3).tolist())
                 keep = ~done
-                sids, req_idx, eligible, arrival, left, nmb = (sids[keep], req_idx[keep], eligible[keep],
-                                                                arrival[keep], left[keep], nmb[keep])
+                sids, req_idx, eligible, arrival, left, nmb, started = (
+                    sids[keep], req_idx[keep], eligible[keep], arrival[keep], left[keep], nmb[keep], started[keep])
         seconds = time.perf_counter() - t_start
         fl = np.concatenate(frame_lat) if frame_lat else np.zeros(1)
         rl = np.array(req_lat) if req_lat else np.zeros(1)
+        iw = np.array(input_wait) if input_wait else np.zeros(1)
         stats = ServingStats(requests=len(requests), frames=frames, cif_equivalent_frames=mbs / 396.0,
                              seconds=seconds, frame_latency_ms_p50=float(np.percentile(fl, 50)),
                              frame_latency_ms_p99=float(np.percentile(fl, 99)),
                              request_latency_ms_p50=float(np.percentile(rl, 50)),
-                             request_latency_ms_p99=float(np.percentile(rl, 99)), max_concurrent=max_conc,
+                             request_latency_ms_p99=float(np.percentile(rl, 99)),
+                             input_wait_ms_p50=float(np.percentile(iw, 50)),
+                             input_wait_ms_p99=float(np.percentile(iw, 99)), max_concurrent=max_conc,
                              steps=steps)
         return (stats, outputs) if collect else stats
 

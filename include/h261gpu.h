@@ -127,6 +127,10 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                     gives each XCD a contiguous run of macroblocks (neighbours
                                     share its L2), a clear bit has workgroup i code macroblock i.
                                     Default 2. */
+#define H261G_OPT_ASYNC_INPUT 5  /* 1: h261g_stream_generate_input on a new stream generates on a
+                                    second HIP stream, overlapped with frame steps; the stream
+                                    joins the first step after its input is ready.  0 (default):
+                                    the input is ready for the next step */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */
@@ -155,6 +159,10 @@ H261G_API int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_
  * ends (the stream stays open, inactive, afterwards) */
 H261G_API int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_content_params_t *c, int32_t n_frames);
 /* number of frames a stream has coded (synchronises) */
+/* 1 while a stream's asynchronous input (H261G_OPT_ASYNC_INPUT) is still
+ * being generated, 0 once it takes part in frame steps (from the step call
+ * that activated it); host state only, no device access */
+H261G_API int h261g_stream_input_pending(h261g_engine_t *e, int32_t id);
 H261G_API int h261g_stream_frames_done(h261g_engine_t *e, int32_t id);
 /* re-packs the macroblock index space after streams closed (synchronises) */
 H261G_API int h261g_engine_compact(h261g_engine_t *e);

@@ -25,12 +25,13 @@ def alone(r: Request):
     return out
 
 
-def test_serving_mix_equals_alone(oracle):
+@pytest.mark.parametrize("async_input", [True, False], ids=["async-input", "sync-input"])
+def test_serving_mix_equals_alone(oracle, async_input):
     reqs = poisson_requests(rate=400.0, duration=0.05, seed=11, median_frames=6, sigma=0.5, cap=12)
     assert len(reqs) >= 8
-    announce(f"serving {len(reqs)} requests")
+    announce(f"serving {len(reqs)} requests, async input {async_input}")
     # a tight budget forces admission to wait for memory
-    loop = ServingLoop(hbm_budget=6 * (1 << 20) * 3, max_streams=16)
+    loop = ServingLoop(hbm_budget=6 * (1 << 20) * 3, max_streams=16, async_input=async_input)
     stats, outs = loop.run(reqs, collect=True, realtime=False)
     loop.close()
     assert stats.frames == sum(r.frames for r in reqs)
