@@ -381,60 +381,47 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
     const int c_filter = search ? wave_sum(t_filter) : 0;
     const int c_intra = wave_sum(t_intra);
 
+    /* h261_mode_decision (src/h261_moded.c:31-100), on wave-uniform values */
+    const bool md_intra = d.frame_type == H261_FRAME_TYPE_INTRA || (c_intra <= c_inter && c_intra <= c_filter);
+    const bool md_filter = !md_intra && c_filter < c_inter;
+    int md_flags = md_filter ? (H261_MB_MC | H261_MB_FILTER) : 0;
+    if (!md_intra && (mvx != 0 || mvy != 0)) md_flags |= H261_MB_MC;
+    const int md_cost = md_intra ? c_intra : (md_filter ? c_filter : c_inter);
+
+    /* results: MeResult (5 dwords) from lanes 0-4, MbMode (6 dwords) from
+     * lanes 8-13 */
+    if (lane < 5) {
+        const uint32_t mv = (uint32_t)(uint8_t)(int8_t)mvx | ((uint32_t)(uint8_t)(int8_t)mvy << 8);
+        const uint32_t w = lane == 0 ? (uint32_t)c_intra : lane == 1 ? 0u : lane == 2 ? (uint32_t)c_inter
+                         : lane == 3 ? (uint32_t)c_filter : mv;
+        ((uint32_t *)(p.result + g))[lane] = w;
+    }
+    if (p.mode && lane >= 8 && lane < 14) {
+        /* MbMode {type, flags, mvx, mvy, quant_adjust, cost} */
+        const int k = lane - 8;
+        const int w = k == 0 ? (md_intra ? H261_MB_TYPE_INTRA : H261_MB_TYPE_INTER) : k == 1 ? md_flags
+                    : k == 2 ? (md_intra ? 0 : mvx) : k == 3 ? (md_intra ? 0 : mvy) : k == 4 ? 0 : md_cost;
+        ((int32_t *)(p.mode + g))[k] = w;
+    }
     if (lane == 0) {
-        MeResult r;
-        r.intra_cost = c_intra;
-        r.zero_cost = 0;
-        r.inter_cost = c_inter;
-        r.filter_cost = c_filter;
-        r.mvx = (int8_t)mvx;
-        r.mvy = (int8_t)mvy;
-        r.pad = 0;
-        p.result[g] = r;
         if (p.mv_start) {
             p.mv_start[2 * g] = (int8_t)mvx;
             p.mv_start[2 * g + 1] = (int8_t)mvy;
         }
-        if (p.mode) {
-            /* h261_mode_decision (src/h261_moded.c:31-100) */
-            MbMode md;
-            md.quant_adjust = 0;
-            bool intra = d.frame_type == H261_FRAME_TYPE_INTRA ||
-                         (c_intra <= c_inter && c_intra <= c_filter);
-            if (intra) {
-                md.type = H261_MB_TYPE_INTRA;
-                md.cost = c_intra;
-                md.flags = 0;
-                md.mvx = md.mvy = 0;
+        if (p.mode && p.rc) {
+            if (p.fuse_tail) {
+                /* count and cost sum in one atomic: the MB that completes
+                 * the count holds the frame's cost sum and starts the picture
+                 * (rate control, quantiser) */
+                RcState &rc = p.rc[s];
+                const unsigned long long old =
+                    atomicAdd((unsigned long long *)&rc.me_acc, (1ull << 32) | (uint32_t)md_cost);
+                if ((int)(old >> 32) == d.mb_w * d.mb_h - 1) {
+                    fc::rc_begin_one(rc, d, (int)(uint32_t)old + md_cost);
+                    atomicExch((unsigned long long *)&rc.me_acc, 0ull);
+                }
             } else {
-                md.type = H261_MB_TYPE_INTER;
-                if (c_filter < c_inter) {
-                    md.flags = H261_MB_MC | H261_MB_FILTER;
-                    md.cost = c_filter;
-                } else {
-                    md.flags = 0;
-                    md.cost = c_inter;
-                }
-                md.mvx = mvx;
-                md.mvy = mvy;
-                if (mvx != 0 || mvy != 0) md.flags |= H261_MB_MC;
-            }
-            p.mode[g] = md;
-            if (p.rc) {
-                if (p.fuse_tail) {
-                    /* count and cost sum in one atomic: the MB that completes
-                     * the count holds the frame's cost sum and starts the
-                     * picture (rate control, quantiser) */
-                    RcState &rc = p.rc[s];
-                    const unsigned long long old =
-                        atomicAdd((unsigned long long *)&rc.me_acc, (1ull << 32) | (uint32_t)md.cost);
-                    if ((int)(old >> 32) == d.mb_w * d.mb_h - 1) {
-                        fc::rc_begin_one(rc, d, (int)(uint32_t)old + md.cost);
-                        atomicExch((unsigned long long *)&rc.me_acc, 0ull);
-                    }
-                } else {
-                    atomicAdd(&p.rc[s].cost_accum, md.cost);
-                }
+                atomicAdd(&p.rc[s].cost_accum, md_cost);
             }
         }
     }
