@@ -57,6 +57,7 @@ OPT_MBS_PER_WAVE = 2
 OPT_FUSED_TAILS = 3
 OPT_XCD_REMAP = 4
 OPT_ASYNC_INPUT = 5
+OPT_DEVICE_BITSTREAM = 6
 
 
 class StreamParams(ctypes.Structure):
@@ -121,6 +122,7 @@ def load() -> ctypes.CDLL:
         "h261g_stream_read_input": (ctypes.c_int, [P, i32, P, P, P]),
         "h261g_engine_profile": (ctypes.c_int, [P, i32]),
         "h261g_engine_kernel_times": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double),
+                                                      ctypes.POINTER(ctypes.c_double),
                                                       ctypes.POINTER(ctypes.c_double),
                                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
         "h261g_content_frame_host": (ctypes.c_int, [ctypes.POINTER(ContentParams), i32, i32, i32, P, P, P]),

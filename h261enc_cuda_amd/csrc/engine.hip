@@ -57,7 +57,9 @@ struct Slot {
     uint8_t *input = nullptr;    /* host-fed frame buffer (I420) */
     uint8_t *recon = nullptr;    /* two I420 buffers */
     uint8_t *seq = nullptr;      /* device frame sequence */
-    size_t input_cap = 0, recon_cap = 0, seq_cap = 0; /* allocation sizes (device memory cache) */
+    uint8_t *bits = nullptr;     /* the last coded frame's bitstream (device writer) */
+    size_t input_cap = 0, recon_cap = 0, seq_cap = 0, bits_cap = 0; /* allocation sizes (device memory cache) */
+    int bits_words = 0;          /* usable capacity of bits in 32-bit words */
     int seq_frames = 0;
     int64_t footprint = 0;
     bool pending = false;        /* host frame queued for the next step */
@@ -152,6 +154,10 @@ struct h261g_engine {
      * time (that kernel is VALU/LDS bound, its re-reads are absorbed below
      * L2), so only the coder is remapped by default */
     int xcd_remap = 2;
+    /* 1: bitstreams written on the device into a per-stream buffer of
+     * kBitsBytes*; 2: buffers of the worst-case frame size; 0: written on
+     * the host from the levels when read */
+    int device_bits = 1;
     hipStream_t stream = nullptr;
     std::vector<Slot> slots;
     /* device arrays indexed by stream slot */
@@ -166,6 +172,7 @@ struct h261g_engine {
     MbMode *mode = nullptr;
     MbSide *side = nullptr;
     int8_t *levels = nullptr; /* [mb][6][64]; intra DC levels (1..255) as uint8 */
+    MbHdr *hdr = nullptr;     /* [mb]: bitstream places, frame tail -> writer (all invalid between steps) */
     std::vector<int> free_cif, free_qcif; /* MB ranges of closed streams */
     h261_macroblock_t *mbstruct = nullptr; /* scratch for read_macroblocks */
     int mbstruct_cap = 0;
@@ -179,19 +186,34 @@ struct h261g_engine {
      * host keeps enqueueing ahead of the device and no host gap lands
      * between an event and the kernel it brackets */
     static constexpr int kEvRing = 64;
-    hipEvent_t ev[kEvRing][6] = {}; /* start/stop of ME, MB coder, frame tail */
+    hipEvent_t ev[kEvRing][8] = {}; /* start/stop of ME, MB coder, frame tail, bitstream writer */
     int64_t ev_recorded = 0, ev_read = 0;
-    double me_ms = 0, mb_ms = 0, other_ms = 0;
+    double me_ms = 0, mb_ms = 0, other_ms = 0, bits_ms = 0;
     int64_t launches = 0;
 };
 
 namespace {
 
-int64_t stream_footprint(int w, int h)
+/* bitstream buffer of one stream: four times the largest coded picture
+ * H.261 allows (256 kbit CIF, 64 kbit QCIF); a larger frame is written on
+ * the host from its levels when read.  mode 2: the worst case of any
+ * frame, so the device always writes it. */
+constexpr int64_t kBitsBytesCif = 128 * 1024, kBitsBytesQcif = 32 * 1024;
+int64_t bits_bytes(int w, int h, int mode)
+{
+    if (mode == 0) return 0;
+    const int n_mb = (w / 16) * (h / 16);
+    if (mode == 2) return ((frame_bits_bound(n_mb) + 31) / 32) * 4;
+    return n_mb == 396 ? kBitsBytesCif : kBitsBytesQcif;
+}
+
+int64_t stream_footprint(int w, int h, int bits_mode)
 {
     const int64_t fsz = (int64_t)w * h * 3 / 2, n_mb = (int64_t)(w / 16) * (h / 16);
-    /* input + two reconstructions + per-MB arrays */
-    return 3 * fsz + n_mb * ((int64_t)sizeof(int32_t) + sizeof(MeResult) + sizeof(MbMode) + sizeof(MbSide) + 768);
+    /* input + two reconstructions + bitstream + per-MB arrays */
+    return 3 * fsz + bits_bytes(w, h, bits_mode) +
+           n_mb * ((int64_t)sizeof(int32_t) + sizeof(MeResult) + sizeof(MbMode) + sizeof(MbSide) + sizeof(MbHdr) +
+                   768);
 }
 
 bool valid_size(int w, int h) { return (w == 352 && h == 288) || (w == 176 && h == 144); }
@@ -208,7 +230,9 @@ int ensure_mb_capacity(h261g_engine *e, int n)
     MbMode *mo = nullptr;
     MbSide *sd = nullptr;
     int8_t *lv = nullptr;
+    MbHdr *hd = nullptr;
     if (!ok(hipMalloc((void **)&ms, sizeof(int32_t) * cap), "hipMalloc") ||
+        !ok(hipMalloc((void **)&hd, sizeof(MbHdr) * cap), "hipMalloc") ||
         !ok(hipMalloc((void **)&me, sizeof(MeResult) * cap), "hipMalloc") ||
         !ok(hipMalloc((void **)&mo, sizeof(MbMode) * cap), "hipMalloc") ||
         !ok(hipMalloc((void **)&sd, sizeof(MbSide) * cap), "hipMalloc") ||
@@ -218,7 +242,18 @@ int ensure_mb_capacity(h261g_engine *e, int n)
         hipFree(mo);
         hipFree(sd);
         hipFree(lv);
+        hipFree(hd);
         return H261G_ERR_MEMORY;
+    }
+    /* no MB has a pending bitstream place between steps */
+    if (!ok(hipMemset(hd, 0, sizeof(MbHdr) * cap), "hipMemset")) {
+        hipFree(ms);
+        hipFree(me);
+        hipFree(mo);
+        hipFree(sd);
+        hipFree(lv);
+        hipFree(hd);
+        return H261G_ERR_DEVICE;
     }
     /* the map and the last step's results of the MBs in use survive growth */
     if (e->mb_stream && e->mb_used) {
@@ -234,7 +269,9 @@ int ensure_mb_capacity(h261g_engine *e, int n)
     hipFree(e->mode);
     hipFree(e->side);
     hipFree(e->levels);
+    hipFree(e->hdr);
     e->mb_stream = ms;
+    e->hdr = hd;
     e->me = me;
     e->mode = mo;
     e->side = sd;
@@ -331,8 +368,11 @@ int enqueue_step(h261g_engine *e, bool timed)
     bl.mbs_per_wave = e->mbs_per_wave ? e->mbs_per_wave : (n_mb >= 4 * kFillWaves ? 4 : (n_mb >= 2 * kFillWaves ? 2 : 1));
     bl.xcd_remap = (e->xcd_remap >> 1) & 1;
     if (launch_mb(bl, st, timed ? ev[2] : nullptr, timed ? ev[3] : nullptr)) return H261G_ERR_DEVICE;
+    /* with device bitstreams the frame tail also places every coded MB in
+     * its frame's bitstream and writes the picture and GOB headers */
+    MbHdr *hdr = e->device_bits ? e->hdr : nullptr;
     if (fused) {
-        if (launch_frame_tail(e->desc, e->seq, e->dn, e->rc, e->side, ns, st, timed ? ev[4] : nullptr,
+        if (launch_frame_tail(e->desc, e->seq, e->dn, e->rc, e->side, hdr, ns, st, timed ? ev[4] : nullptr,
                               timed ? ev[5] : nullptr))
             return H261G_ERR_DEVICE;
     } else {
@@ -340,9 +380,23 @@ int enqueue_step(h261g_engine *e, bool timed)
          * (rate-control begin, between the two big kernels, is not timed) */
         if (timed) (void)hipEventRecord(ev[4], st);
         if (launch_denoise_update(e->dn, e->desc, ns, st)) return H261G_ERR_DEVICE;
-        if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st)) return H261G_ERR_DEVICE;
+        if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st, e->seq, hdr)) return H261G_ERR_DEVICE;
         if (launch_advance(e->desc, e->seq, ns, st)) return H261G_ERR_DEVICE;
         if (timed) (void)hipEventRecord(ev[5], st);
+    }
+    if (hdr) {
+        BitLaunch wl{};
+        wl.mb_stream = e->mb_stream;
+        wl.n_mb = n_mb;
+        wl.seq = e->seq;
+        wl.hdr = hdr;
+        wl.side = e->side;
+        wl.levels = e->levels;
+        wl.xcd_remap = (e->xcd_remap >> 1) & 1;
+        if (launch_bitstream(wl, st, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr)) return H261G_ERR_DEVICE;
+    } else if (timed) {
+        (void)hipEventRecord(ev[6], st);
+        (void)hipEventRecord(ev[7], st);
     }
     if (timed) e->ev_recorded++;
     return H261G_OK;
@@ -352,14 +406,16 @@ int enqueue_step(h261g_engine *e, bool timed)
 int read_oldest_times(h261g_engine *e)
 {
     hipEvent_t *ev = e->ev[e->ev_read % h261g_engine::kEvRing];
-    if (!ok(hipEventSynchronize(ev[5]), "event")) return H261G_ERR_DEVICE;
-    float a = 0, c = 0, f = 0;
+    if (!ok(hipEventSynchronize(ev[7]), "event")) return H261G_ERR_DEVICE;
+    float a = 0, c = 0, f = 0, b = 0;
     (void)hipEventElapsedTime(&a, ev[0], ev[1]);
     (void)hipEventElapsedTime(&c, ev[2], ev[3]);
     (void)hipEventElapsedTime(&f, ev[4], ev[5]);
+    (void)hipEventElapsedTime(&b, ev[6], ev[7]);
     e->me_ms += a;
     e->other_ms += f;
     e->mb_ms += c;
+    e->bits_ms += b;
     e->launches++;
     e->ev_read++;
     return H261G_OK;
@@ -408,6 +464,13 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         if (value != 0 && value != 1) return H261G_ERR_ARG;
         e->async_input = value;
         return H261G_OK;
+    case H261G_OPT_DEVICE_BITSTREAM:
+        if (value < 0 || value > 2) return H261G_ERR_ARG;
+        /* bitstream buffers are sized at open */
+        for (const auto &sl : e->slots)
+            if (sl.open) return H261G_ERR_STATE;
+        e->device_bits = value;
+        break;
     default:
         return H261G_ERR_ARG;
     }
@@ -435,7 +498,7 @@ int h261g_device_info(h261g_device_info_t *out)
 int64_t h261g_stream_footprint(int32_t width, int32_t height)
 {
     if (!valid_size(width, height)) return H261G_ERR_ARG;
-    return stream_footprint(width, height);
+    return stream_footprint(width, height, 1);
 }
 
 int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_budget, int32_t mbs_per_wave)
@@ -459,7 +522,7 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipMemset(e->desc, 0, sizeof(StreamDesc) * max_streams), "hipMemset") &&
                 ok(hipMemset(e->seq, 0, sizeof(SeqState) * max_streams), "hipMemset");
     for (int r = 0; good && r < h261g_engine::kEvRing; r++)
-        for (int i = 0; good && i < 6; i++) good = ok(hipEventCreate(&e->ev[r][i]), "event");
+        for (int i = 0; good && i < 8; i++) good = ok(hipEventCreate(&e->ev[r][i]), "event");
     if (!good) {
         h261g_engine_destroy(e);
         return H261G_ERR_DEVICE;
@@ -478,6 +541,7 @@ int h261g_engine_destroy(h261g_engine_t *e)
         hipFree(s.input);
         hipFree(s.recon);
         hipFree(s.seq);
+        hipFree(s.bits);
     }
     for (auto &kv : e->cache) hipFree(kv.second);
     e->cache.clear();
@@ -495,9 +559,10 @@ int h261g_engine_destroy(h261g_engine_t *e)
     hipFree(e->mode);
     hipFree(e->side);
     hipFree(e->levels);
+    hipFree(e->hdr);
     hipHostFree(e->mbstruct);
     for (int r = 0; r < h261g_engine::kEvRing; r++)
-        for (int i = 0; i < 6; i++)
+        for (int i = 0; i < 8; i++)
             if (e->ev[r][i]) (void)hipEventDestroy(e->ev[r][i]);
     if (e->compute_ev) (void)hipEventDestroy(e->compute_ev);
     if (e->in_stream) (void)hipStreamDestroy(e->in_stream);
@@ -518,7 +583,7 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
             break;
         }
     if (id < 0) return H261G_ERR_FULL;
-    const int64_t fp = stream_footprint(p->width, p->height);
+    const int64_t fp = stream_footprint(p->width, p->height, e->device_bits);
     if (e->budget && e->used + fp > e->budget) return H261G_ERR_FULL;
     Slot &s = e->slots[id];
     const int64_t fsz = (int64_t)p->width * p->height * 3 / 2;
@@ -533,8 +598,11 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
     s.params = *p;
     s.input = cache_get(e, fsz, &s.input_cap);
     s.recon = s.input ? cache_get(e, 2 * fsz, &s.recon_cap) : nullptr;
-    if (!s.input || !s.recon) {
+    const int64_t bbytes = bits_bytes(p->width, p->height, e->device_bits);
+    s.bits = (s.recon && bbytes) ? cache_get(e, (size_t)bbytes, &s.bits_cap) : nullptr;
+    if (!s.input || !s.recon || (bbytes && !s.bits)) {
         cache_put(e, s.input, s.input_cap);
+        cache_put(e, s.recon, s.recon_cap);
         s = Slot();
         s.gen_ev = gen_ev;
         return H261G_ERR_MEMORY;
@@ -551,6 +619,7 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
         if (rcode) {
             cache_put(e, s.input, s.input_cap);
             cache_put(e, s.recon, s.recon_cap);
+            cache_put(e, s.bits, s.bits_cap);
             s = Slot();
             s.gen_ev = gen_ev;
             return rcode;
@@ -590,6 +659,9 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
     q.pp = 0;
     q.recon[0] = s.recon;
     q.recon[1] = s.recon + fsz;
+    s.bits_words = (int)(bbytes / 4);
+    q.bits = (uint32_t *)s.bits;
+    q.bits_words = s.bits_words;
     DenoiseState dn{};
     dn.denoise = p->denoise;
     RcState rc{};
@@ -626,6 +698,7 @@ int h261g_stream_close(h261g_engine_t *e, int32_t id)
     cache_put(e, s->input, s->input_cap);
     cache_put(e, s->recon, s->recon_cap);
     cache_put(e, s->seq, s->seq_cap);
+    cache_put(e, s->bits, s->bits_cap);
     e->used -= s->footprint;
     (s->n_mb == 396 ? e->free_cif : e->free_qcif).push_back(s->mb_base);
     *s = Slot();
@@ -952,6 +1025,17 @@ int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int
 {
     Slot *s = get_slot(e, id);
     if (!s || !out || cap < 0) return H261G_ERR_ARG;
+    if (s->bits) {
+        /* written by the device when the frame fit the stream's buffer */
+        RcState rc;
+        if (!ok(hipMemcpy(&rc, e->rc + id, sizeof(rc), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
+        if (rc.frame_bits > 0 && (rc.frame_bits + 31) / 32 <= s->bits_words) {
+            if (rc.frame_bytes > cap) return H261G_ERR_FULL;
+            if (!ok(hipMemcpy(out, s->bits, (size_t)rc.frame_bytes, hipMemcpyDeviceToHost), "hipMemcpy"))
+                return H261G_ERR_DEVICE;
+            return rc.frame_bytes;
+        }
+    }
     if (e->mbstruct_cap < s->n_mb) {
         hipHostFree(e->mbstruct);
         e->mbstruct = nullptr;
@@ -974,12 +1058,13 @@ int h261g_engine_profile(h261g_engine_t *e, int32_t enable)
     if (e->ev_read < e->ev_recorded && !ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
     e->ev_read = e->ev_recorded;
     e->profile = enable != 0;
-    e->me_ms = e->mb_ms = e->other_ms = 0;
+    e->me_ms = e->mb_ms = e->other_ms = e->bits_ms = 0;
     e->launches = 0;
     return H261G_OK;
 }
 
-int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, double *other_ms, int64_t *launches)
+int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, double *other_ms, double *bits_ms,
+                              int64_t *launches)
 {
     if (!e) return H261G_ERR_ARG;
     int rcode = read_all_times(e);
@@ -987,6 +1072,7 @@ int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, d
     if (me_ms) *me_ms = e->me_ms;
     if (mb_ms) *mb_ms = e->mb_ms;
     if (other_ms) *other_ms = e->other_ms;
+    if (bits_ms) *bits_ms = e->bits_ms;
     if (launches) *launches = e->launches;
     return H261G_OK;
 }

@@ -47,8 +47,21 @@ __constant__ int32_t c_qscale[63] = {
 __constant__ Vlc c_mba[35] = H261G_MBA_TABLE;
 __constant__ Vlc c_mtype[10] = H261G_MTYPE_TABLE;
 __constant__ Vlc c_mvd[32] = H261G_MVD_TABLE;
-/* MTYPE index by [MQUANT coded][flags MC|FILTER|COEFF] (macroblock.c:131-150) */
-__constant__ int8_t c_mtype_of_flags[2][8] = {{-1, 4, -1, 7, 2, 5, -1, 8}, {-1, -1, -1, -1, 3, 6, -1, 9}};
+__constant__ Vlc c_cbp[64] = H261G_CBP_TABLE;
+/* MTYPE index by [MQUANT coded][flags MC|FILTER|COEFF] (macroblock.c:131-150),
+ * -1 where no MTYPE exists; packed as value + 1 in 4-bit fields so the lookup
+ * is a shift of a constant */
+constexpr uint64_t kMtypeOfFlags = [] {
+    constexpr int8_t t[2][8] = {{-1, 4, -1, 7, 2, 5, -1, 8}, {-1, -1, -1, -1, 3, 6, -1, 9}};
+    uint64_t v = 0;
+    for (int q = 0; q < 2; q++)
+        for (int f = 0; f < 8; f++) v |= (uint64_t)(t[q][f] + 1) << (4 * (8 * q + f));
+    return v;
+}();
+__device__ __forceinline__ int mtype_of_flags(bool coded_q, int flags)
+{
+    return (int)((kMtypeOfFlags >> (4 * (8 * (coded_q ? 1 : 0) + flags))) & 15u) - 1;
+}
 
 constexpr int kRcBits = 6;
 constexpr int kRcOne = 1 << kRcBits;
@@ -118,10 +131,54 @@ __device__ __forceinline__ void rc_update_one(RcState &rc, int ft, int bytes)
     rc.vbv_fill -= bytes;
 }
 
+/* inclusive prefix sum over the wave's lanes: a scan within each row of 16
+ * lanes by DPP row shifts (lanes shifted in from outside the row read 0),
+ * then the totals of the rows below added from three readlanes.  VALU
+ * latency only: no LDS round trip as a lane shuffle would take. */
+__device__ __forceinline__ int wave_inclusive_sum(int x, int lane)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true); /* row_shr:1 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true); /* row_shr:2 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true); /* row_shr:4 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true); /* row_shr:8 */
+    const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31),
+              r2 = __builtin_amdgcn_readlane(x, 47);
+    const int row = lane >> 4;
+    return x + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+}
+
+/* exclusive prefix sum over the wave's lanes */
+__device__ __forceinline__ int wave_exclusive_sum(int v, int lane) { return wave_inclusive_sum(v, lane) - v; }
+
+/* ORs the low `len` (<= 32) bits of code into a bitstream of 32-bit words at
+ * bit position pos.  Words hold stream bytes in order (the first bit of the
+ * stream is the most significant bit of byte 0), so the big-endian bit
+ * layout is byte-swapped before the OR; neighbours share boundary words. */
+__device__ __forceinline__ void or_bits(uint32_t *words, int pos, uint32_t code, int len)
+{
+    if (len <= 0) return;
+    const int i = pos >> 5, off = pos & 31;
+    if (off + len <= 32) {
+        atomicOr(words + i, __builtin_bswap32(code << (32 - off - len)));
+    } else {
+        const int spill = off + len - 32;
+        atomicOr(words + i, __builtin_bswap32(code >> spill));
+        atomicOr(words + i + 1, __builtin_bswap32(code << (32 - spill)));
+    }
+}
+
 /* all 64 lanes call; returns the frame's total bits on every lane.  The side
  * information of every GOB is loaded before any is scanned, so the loads
- * overlap instead of costing one memory latency per GOB. */
-__device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide *side, int picq, int lane)
+ * overlap instead of costing one memory latency per GOB, and the code tables
+ * are copied to LDS first, so no lookup inside the scan waits on memory.
+ *
+ * With hdr and bits (the stream's bitstream words, at least the frame's
+ * size): each coded MB's bit position and header codes (MBA, MTYPE, MQUANT,
+ * MVD, CBP) go to hdr for the bitstream writer, the frame's words are zeroed
+ * and the picture and GOB headers written (src/h261_picture.c:38-71,
+ * src/h261_gob.c:58-199, src/h261_macroblock.c:94-245). */
+__device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide *side, int picq, int lane,
+                                               MbHdr *hdr = nullptr, uint32_t *bits = nullptr, int bits_words = 0)
 {
     constexpr int kMaxGob = 12;
     const bool cif = d.mb_w == 22;
@@ -146,7 +203,23 @@ __device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide
         const uint64_t v = (valid && g < n_gob) ? raw[g] : 0ull;
         __builtin_memcpy(&sds[g], &v, sizeof(v));
     }
-    int bits = 0;
+    /* MBA, MTYPE, MVD and CBP codes: code | len << 16 */
+    __shared__ uint32_t t_mba[35], t_mtype[10], t_mvd[32], t_cbp[64];
+    {
+        const uint32_t *g_mba = (const uint32_t *)c_mba, *g_mtype = (const uint32_t *)c_mtype,
+                       *g_mvd = (const uint32_t *)c_mvd, *g_cbp = (const uint32_t *)c_cbp;
+        const uint32_t a = g_mba[lane < 35 ? lane : 0], b = g_mtype[lane < 10 ? lane : 0],
+                       c = g_mvd[lane < 32 ? lane : 0], e = g_cbp[lane];
+        if (lane < 35) t_mba[lane] = a;
+        if (lane < 10) t_mtype[lane] = b;
+        if (lane < 32) t_mvd[lane] = c;
+        t_cbp[lane] = e;
+        __syncthreads();
+    }
+    auto vcode = [](uint32_t e) { return e & 0xffffu; };
+    auto vlen = [](uint32_t e) { return (int)((e >> 16) & 0xffu); };
+    int pos = kPictureHeaderBits; /* bit position in the frame */
+    int my_gob_pos = 0;           /* lane g: where GOB g's header starts */
 #pragma unroll
     for (int g = 0; g < kMaxGob; g++) {
         if (g >= n_gob) break;
@@ -170,30 +243,76 @@ __device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide
         const int pmvx = __shfl((int)sd.mvx, src, 64), pmvy = __shfl((int)sd.mvy, src, 64);
         const bool mv_valid = lane % 11 != 0 && (pflags & H261_MB_MC);
         int b = 0;
+        uint64_t hc = 0; /* header codes of the MB, first bit most significant */
+        int hl = 0;
+        auto app = [&](uint32_t c, int l) {
+            hc = (hc << l) | c;
+            hl += l;
+        };
         if (coded) {
-            b += c_mba[addr].len;
+            const uint32_t ea = t_mba[addr];
+            app(vcode(ea), vlen(ea));
             if (sd.type == H261_MB_TYPE_INTRA) {
-                b += sd.quant != cq ? c_mtype[1].len + 5 : c_mtype[0].len;
+                const uint32_t et = t_mtype[sd.quant != cq ? 1 : 0];
+                app(vcode(et), vlen(et));
+                if (sd.quant != cq) app((uint32_t)sd.quant, 5);
+                b = hl + sd.bits;
             } else {
                 const bool coded_q = sd.quant != cq && (flags & H261_MB_COEFF);
-                const int t = c_mtype_of_flags[coded_q ? 1 : 0][flags & 7];
-                b += (t >= 0 ? c_mtype[t].len : 0) + (coded_q ? 5 : 0);
+                const int t = mtype_of_flags(coded_q, flags & 7);
+                if (t >= 0) app(vcode(t_mtype[t]), vlen(t_mtype[t]));
+                if (coded_q) app((uint32_t)sd.quant, 5);
                 if (flags & H261_MB_MC) {
                     int dx = sd.mvx, dy = sd.mvy;
                     if (mv_valid) {
                         dx -= pmvx;
                         dy -= pmvy;
                     }
-                    b += c_mvd[(dx + 48) & 31].len + c_mvd[(dy + 48) & 31].len;
+                    const uint32_t vx = t_mvd[(dx + 48) & 31], vy = t_mvd[(dy + 48) & 31];
+                    app(vcode(vx), vlen(vx));
+                    app(vcode(vy), vlen(vy));
+                }
+                b = hl + sd.bits; /* sd.bits: CBP + TCOEFF */
+                if (flags & H261_MB_COEFF) {
+                    const uint32_t ec = t_cbp[(uint8_t)sd.cbp & 63];
+                    app(vcode(ec), vlen(ec));
                 }
             }
-            b += sd.bits;
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) b += __shfl_xor(b, off, 64);
-        bits += b + kGobHeaderBits;
+        const int gob_pos = pos;
+        if (lane == g) my_gob_pos = gob_pos;
+        const int incl = wave_inclusive_sum(b, lane);
+        if (hdr && coded) {
+            const int gx = (g % gob_cols) * 11, gy = (g / gob_cols) * 3;
+            MbHdr h;
+            h.code = hc;
+            h.bitpos = gob_pos + kGobHeaderBits + incl - b;
+            h.len = (int16_t)hl;
+            h.valid = 1;
+            hdr[d.mb_base + (gy + lane / 11) * d.mb_w + gx + lane % 11] = h;
+        }
+        pos = gob_pos + kGobHeaderBits + __builtin_amdgcn_readlane(incl, 63);
     }
-    return bits + kPictureHeaderBits;
+    if (bits && ((pos + 31) >> 5) <= bits_words) {
+        /* the frame's words start at zero; then the headers (the MB bits are
+         * ORed in by the writer launch) */
+        const int words = (pos + 31) >> 5;
+        for (int i = lane; i < words; i += 64) bits[i] = 0u;
+        __syncthreads();
+        if (lane == 0) {
+            /* PSC, TR, PTYPE (split, document camera, freeze release, source
+             * format, hi-res and spare bits as the reference sets them), PEI */
+            const uint32_t ph = (16u << 12) | ((uint32_t)(d.frame_num & 31) << 7) | (1u << 4) | ((cif ? 1u : 0u) << 3) |
+                                (1u << 2) | (1u << 1);
+            bits[0] = __builtin_bswap32(ph);
+        }
+        if (lane < n_gob) {
+            /* GBSC, GN, GQUANT, GEI */
+            const uint32_t gn = cif ? lane + 1 : 2 * lane + 1;
+            or_bits(bits, my_gob_pos, (1u << 10) | (gn << 6) | ((uint32_t)picq << 1), kGobHeaderBits);
+        }
+    }
+    return pos;
 }
 
 /* 64 lanes of one wave; the caller provides a workgroup barrier between the
@@ -239,11 +358,13 @@ __device__ __forceinline__ void advance_one(StreamDesc &d, SeqState &q)
  * for the next frame, exact frame size, rate-control update, advance.  Reads
  * only what earlier launches wrote (statistics, side information). */
 __device__ __forceinline__ void frame_tail_wave(StreamDesc &d, SeqState &q, DenoiseState *dn, RcState &rc,
-                                                const MbSide *side, int32_t *bytes_out, int lane)
+                                                const MbSide *side, MbHdr *hdr, int32_t *bytes_out, int lane)
 {
     const int ci = dn->inter_count, ca = dn->intra_count;
     denoise_update_lane(dn, lane, ci, ca);
-    const int bits = frame_bits_wave(d, side, rc.pic_quant, lane);
+    const bool write = hdr && q.bits;
+    const int bits = frame_bits_wave(d, side, rc.pic_quant, lane, write ? hdr : nullptr, write ? q.bits : nullptr,
+                                     q.bits_words);
     __syncthreads();
     if (lane == 0) {
         dn->inter_count = 0;

@@ -26,13 +26,16 @@ __global__ void rc_begin_kernel(RcState *rc_all, const StreamDesc *streams, int 
 }
 
 __global__ __launch_bounds__(64) void frame_bits_kernel(RcState *rc_all, const StreamDesc *streams, int n,
-                                                         const MbSide *side, int32_t *bytes_out)
+                                                         const MbSide *side, int32_t *bytes_out, const SeqState *seq,
+                                                         MbHdr *hdr)
 {
     const int s = blockIdx.x;
     const StreamDesc &d = streams[s];
     if (!(d.flags & kStreamActive)) return;
     RcState &rc = rc_all[s];
-    const int bits = fc::frame_bits_wave(d, side, rc.pic_quant, threadIdx.x);
+    const bool write = hdr && seq && seq[s].bits;
+    const int bits = fc::frame_bits_wave(d, side, rc.pic_quant, threadIdx.x, write ? hdr : nullptr,
+                                         write ? seq[s].bits : nullptr, write ? seq[s].bits_words : 0);
     if (threadIdx.x == 0) {
         const int bytes = (bits + 7) >> 3;
         fc::rc_update_one(rc, d.frame_type, bytes);
@@ -57,12 +60,14 @@ __global__ __launch_bounds__(64) void denoise_update_kernel(DenoiseState *dn_all
 }
 
 __global__ __launch_bounds__(64) void frame_tail_kernel(StreamDesc *streams, SeqState *seq, DenoiseState *dn_all,
-                                                         RcState *rc_all, const MbSide *side, int32_t *bytes_out)
+                                                         RcState *rc_all, const MbSide *side, MbHdr *hdr,
+                                                         int32_t *bytes_out)
 {
     const int s = blockIdx.x;
     StreamDesc &d = streams[s];
     if (!(d.flags & kStreamActive)) return;
-    fc::frame_tail_wave(d, seq[s], dn_all + s, rc_all[s], side, bytes_out ? bytes_out + s : nullptr, threadIdx.x);
+    fc::frame_tail_wave(d, seq[s], dn_all + s, rc_all[s], side, hdr, bytes_out ? bytes_out + s : nullptr,
+                        threadIdx.x);
 }
 
 __global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
@@ -77,7 +82,7 @@ __global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
 }  // namespace
 
 int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
-                      int n_streams, void *stream, void *ev_start, void *ev_stop)
+                      MbHdr *hdr, int n_streams, void *stream, void *ev_start, void *ev_stop)
 {
     if (n_streams <= 0) {
         if (ev_start) { /* nothing to launch: an empty timed interval */
@@ -89,10 +94,10 @@ int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcSt
     int32_t *no_bytes = nullptr;
     if (ev_start)
         hipExtLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream,
-                              (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0, streams, seq, dn, rc, side, no_bytes);
+                              (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0, streams, seq, dn, rc, side, hdr, no_bytes);
     else
         hipLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, streams, seq, dn, rc,
-                           side, no_bytes);
+                           side, hdr, no_bytes);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -113,11 +118,11 @@ int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int, 
 }
 
 int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, const MbSide *side, int32_t *bytes_out,
-                      void *stream)
+                      void *stream, const SeqState *seq, MbHdr *hdr)
 {
     if (n_streams <= 0) return 0;
     hipLaunchKernelGGL(frame_bits_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, rc, streams, n_streams,
-                       side, bytes_out);
+                       side, bytes_out, seq, hdr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -55,7 +55,24 @@ struct SeqState {
     int32_t remaining;  /* frames left including the current one */
     int32_t pp;         /* which reconstruction buffer is the reference */
     uint8_t *recon[2];
+    uint32_t *bits;     /* optional: the coded frame's bitstream (bytes in stream order) */
+    int32_t bits_words; /* capacity of bits in 32-bit words */
+    int32_t pad;
 };
+
+/* one coded MB's place in its frame's bitstream, from the frame tail to the
+ * bitstream writer */
+struct MbHdr {
+    uint64_t code;  /* MBA, MTYPE, MQUANT, MVD and CBP codes, first bit most significant */
+    int32_t bitpos; /* bit offset of the MBA code in the frame */
+    int16_t len;    /* bits in code */
+    int16_t valid;  /* 1: written by the next writer launch, which clears it */
+};
+static_assert(sizeof(MbHdr) == 16, "MbHdr layout");
+
+/* worst-case bits of one frame: per MB an escape-coded level at every
+ * coefficient of 6 blocks plus headers, plus picture and GOB headers */
+constexpr int64_t frame_bits_bound(int n_mb) { return (int64_t)n_mb * (6 * (8 + 63 * 20 + 2) + 64) + 32 + 12 * 26; }
 
 /* motion estimation result of one MB; same fields as me_gpu_mb_t */
 struct MeResult {
@@ -180,13 +197,27 @@ int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int n
 /* per stream, one wave: GOB scan of the MBA / MVD / MQUANT bits, frame
  * size, rate-control update */
 int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, const MbSide *side,
-                      int32_t *frame_bytes_out, void *stream);
+                      int32_t *frame_bytes_out, void *stream, const SeqState *seq = nullptr, MbHdr *hdr = nullptr);
 /* per stream: the reconstruction becomes the reference, next input frame */
 int launch_advance(StreamDesc *streams, SeqState *seq, int n_streams, void *stream);
 /* per stream, one wave: denoise update, frame size, rate-control update and
- * advance in one launch */
+ * advance in one launch; with hdr, also each coded MB's bitstream position
+ * and header codes, and the picture and GOB headers in SeqState.bits */
 int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
-                      int n_streams, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
+                      MbHdr *hdr, int n_streams, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
+
+/* one wave per MB: the coded MBs' bits (headers from the frame tail, then
+ * the TCOEFF codes of their levels) into their streams' bitstreams */
+struct BitLaunch {
+    const int32_t *mb_stream;
+    int32_t n_mb;
+    const SeqState *seq;
+    MbHdr *hdr;
+    const MbSide *side;
+    const int8_t *levels;
+    int32_t xcd_remap;
+};
+int launch_bitstream(const BitLaunch &p, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
 
 }  // namespace h261g
 

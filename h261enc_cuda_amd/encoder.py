@@ -151,11 +151,11 @@ class Engine:
         check(self.lib.h261g_engine_profile(self.h, int(enable)), "profile")
 
     def kernel_times(self):
-        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        a, b, c, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         n = ctypes.c_int64()
         check(self.lib.h261g_engine_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
-                                                 ctypes.byref(n)), "kernel_times")
-        return {"me_ms": a.value, "mb_ms": b.value, "other_ms": c.value, "steps": n.value}
+                                                 ctypes.byref(d), ctypes.byref(n)), "kernel_times")
+        return {"me_ms": a.value, "mb_ms": b.value, "other_ms": c.value, "bits_ms": d.value, "steps": n.value}
 
     def destroy(self) -> None:
         if self.h:

@@ -131,6 +131,12 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                     second HIP stream, overlapped with frame steps; the stream
                                     joins the first step after its input is ready.  0 (default):
                                     the input is ready for the next step */
+#define H261G_OPT_DEVICE_BITSTREAM 6 /* 1 (default): each step also writes every coded frame's H.261
+                                        bitstream on the device, into a buffer per stream of four
+                                        times the largest picture H.261 allows; 2: buffers of the
+                                        worst-case frame size; 0: no device bitstream (written on the
+                                        host when read).  Set while no stream is open
+                                        (H261G_ERR_STATE otherwise). */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */
@@ -190,9 +196,12 @@ H261G_API int h261g_stream_read_macroblocks(h261g_engine_t *e, int32_t id, h261_
 H261G_API int h261g_stream_read_recon(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
 /* motion estimation results of the last frame (n_mb entries) */
 H261G_API int h261g_stream_read_me(h261g_engine_t *e, int32_t id, me_gpu_mb_t *me);
-/* the last frame's H.261 bitstream written on the host from the device's
- * levels and side information; returns its byte count (equal to
- * h261g_frame_info_t.bytes) or an error */
+/* the last frame's H.261 bitstream; returns its byte count (equal to
+ * h261g_frame_info_t.bytes) or an error.  With H261G_OPT_DEVICE_BITSTREAM
+ * (default) the device wrote it during the step and only its bytes are
+ * copied; otherwise, or when the frame outgrew the stream's bitstream
+ * buffer, it is written here on the host from the device's levels and side
+ * information (the same bytes) */
 H261G_API int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int32_t cap);
 /* the current input planes (for checking device-generated content) */
 H261G_API int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
@@ -201,7 +210,7 @@ H261G_API int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y,
  * (HIP events around each launch; enabled by h261g_engine_profile(e, 1)) */
 H261G_API int h261g_engine_profile(h261g_engine_t *e, int32_t enable);
 H261G_API int h261g_engine_kernel_times(h261g_engine_t *e, double *me_ms, double *mb_ms, double *other_ms,
-                              int64_t *launches);
+                                        double *bits_ms, int64_t *launches);
 
 /* synthetic content on the host (the same pels the device generator makes)
  * for frame `frame` of a stream */
