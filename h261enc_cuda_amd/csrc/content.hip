@@ -76,49 +76,78 @@ __global__ __launch_bounds__(256) void content_kernel(const FrameCtx *ctx, uint8
     }
 }
 
+/* FrameCtx of frames first .. first + n - 1, one workgroup: the same
+ * values as content_frame_ctx.  Global motion accumulates the per-frame
+ * steps of frames 1..t: a running workgroup scan over the frames in chunks
+ * of kCtxThreads, from frame 1. */
+constexpr int kCtxThreads = 256;
+__global__ __launch_bounds__(kCtxThreads) void content_ctx_kernel(h261g_content_params_t c, int width, int height,
+                                                                  int first, int n, FrameCtx *ctx)
+{
+    __shared__ int sx[kCtxThreads], sy[kCtxThreads];
+    const int tid = threadIdx.x;
+    int carry_x = 0, carry_y = 0; /* displacement through the previous chunk */
+    for (int t0 = 0; t0 < first + n; t0 += kCtxThreads) {
+        const int t = t0 + tid;
+        int dx = 0, dy = 0;
+        if (!c.local && t >= 1 && t < first + n) {
+            dx = content::global_step(c.seed, t, 0, c.motion);
+            dy = content::global_step(c.seed, t, 1, c.motion);
+        }
+        /* inclusive scan over the chunk */
+        sx[tid] = dx;
+        sy[tid] = dy;
+        __syncthreads();
+        for (int off = 1; off < kCtxThreads; off <<= 1) {
+            const int ax = tid >= off ? sx[tid - off] : 0, ay = tid >= off ? sy[tid - off] : 0;
+            __syncthreads();
+            sx[tid] += ax;
+            sy[tid] += ay;
+            __syncthreads();
+        }
+        if (t >= first && t < first + n) {
+            FrameCtx f;
+            f.noise_seed = c.seed;
+            f.frame = t;
+            f.sigma = c.sigma;
+            f.local = c.local;
+            f.width = width;
+            f.height = height;
+            f.ox = f.oy = 0;
+            f.age = 0;
+            if (c.local) {
+                const int scene = c.scene_cut > 0 ? t / c.scene_cut : 0;
+                f.tex_seed = content::scene_seed(c.seed, scene);
+                f.age = c.scene_cut > 0 ? t - scene * c.scene_cut : t;
+            } else {
+                f.tex_seed = c.seed;
+                f.ox = carry_x + sx[tid];
+                f.oy = carry_y + sy[tid];
+            }
+            ctx[t - first] = f;
+        }
+        carry_x += sx[kCtxThreads - 1];
+        carry_y += sy[kCtxThreads - 1];
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 /* frames [first, first + n) of a stream into out (I420, frame after frame) */
 int generate_frames_device(const h261g_content_params_t &c, int width, int height, int first, int n, uint8_t *out,
-                           void *stream)
+                           FrameCtx *ctx, void *stream)
 {
     if (n <= 0) return 0;
-    FrameCtx *h = (FrameCtx *)malloc(sizeof(FrameCtx) * n);
-    if (!h) return H261G_ERR_MEMORY;
-    /* incremental displacement sums */
-    FrameCtx f = content_frame_ctx(c, width, height, first);
-    for (int i = 0; i < n; i++) {
-        if (i > 0) {
-            const int t = first + i;
-            if (c.local) {
-                f = content_frame_ctx(c, width, height, t);
-            } else {
-                f.frame = t;
-                f.ox += content::global_step(c.seed, t, 0, c.motion);
-                f.oy += content::global_step(c.seed, t, 1, c.motion);
-            }
-        }
-        h[i] = f;
-    }
-    FrameCtx *d = nullptr;
-    int rc = 0;
-    if (hipMallocAsync((void **)&d, sizeof(FrameCtx) * n, (hipStream_t)stream) != hipSuccess) {
-        free(h);
-        return H261G_ERR_MEMORY;
-    }
-    if (hipMemcpyAsync(d, h, sizeof(FrameCtx) * n, hipMemcpyHostToDevice, (hipStream_t)stream) != hipSuccess)
-        rc = H261G_ERR_DEVICE;
+    /* per-frame parameters on the device, then the frames: two launches,
+     * nothing for the host to wait for */
+    hipLaunchKernelGGL(content_ctx_kernel, dim3(1), dim3(kCtxThreads), 0, (hipStream_t)stream, c, width, height, first,
+                       n, ctx);
+    if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
     const int threads = width * height / 4 + width * height / 16;
-    if (!rc) {
-        hipLaunchKernelGGL(content_kernel, dim3((threads + 255) / 256, n), dim3(256), 0, (hipStream_t)stream, d, out,
-                           width, height, (size_t)width * height * 3 / 2);
-        if (hipGetLastError() != hipSuccess) rc = H261G_ERR_DEVICE;
-    }
-    if (hipFreeAsync(d, (hipStream_t)stream) != hipSuccess && !rc) rc = H261G_ERR_DEVICE;
-    /* the pinned-free host copy must outlive the async upload */
-    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && !rc) rc = H261G_ERR_DEVICE;
-    free(h);
-    return rc;
+    hipLaunchKernelGGL(content_kernel, dim3((threads + 255) / 256, n), dim3(256), 0, (hipStream_t)stream, ctx, out,
+                       width, height, (size_t)width * height * 3 / 2);
+    return hipGetLastError() == hipSuccess ? 0 : H261G_ERR_DEVICE;
 }
 
 }  // namespace h261g

@@ -33,13 +33,14 @@
 #include <vector>
 
 #include "../../include/h261gpu.h"
+#include "content.h"
 #include "frame_control.h"
 #include "h261gpu_internal.h"
 
 namespace h261g {
 
 int generate_frames_device(const h261g_content_params_t &c, int width, int height, int first, int n, uint8_t *out,
-                           void *stream);
+                           content::FrameCtx *ctx, void *stream);
 
 namespace {
 
@@ -129,11 +130,19 @@ using namespace h261g;
 struct h261g_engine {
     int max_streams = 0;
     int64_t budget = 0, used = 0;
-    /* device memory cache of closed streams' buffers (size -> block): open
-     * and close neither allocate nor free nor wait when a block fits.  All
-     * work is on one stream, so a block reused by a later open is only
-     * touched after the work that used it before. */
-    std::multimap<size_t, uint8_t *> cache;
+    /* device memory of stream buffers (inputs, reconstructions, frame
+     * sequences, bitstreams): chunks from hipMalloc carved first-fit, ranges
+     * coalesced when released.  Open, close and input generation neither
+     * allocate nor free nor wait while a chunk has room.  All frame work is
+     * on one stream, so a released range is reused only by work queued after
+     * the work that used it (input generation on the second stream first
+     * waits for the work queued so far). */
+    struct DevChunk {
+        uint8_t *base;
+        size_t size;
+        std::map<size_t, size_t> free; /* offset -> length */
+    };
+    std::vector<DevChunk> chunks;
     /* re-packing: pinned host staging (bases then map), its device copy, and
      * the event after which the host staging may be rewritten */
     int32_t *remap_host = nullptr, *remap_dev = nullptr;
@@ -286,38 +295,84 @@ void invalidate_graph(h261g_engine *e)
     e->graph_valid = false;
 }
 
-void cache_release(h261g_engine *e)
+constexpr size_t kChunkBytes = (size_t)256 << 20;
+constexpr size_t kRangeAlign = 256;
+
+/* chunks with no range in use go back to the device (after the queued work) */
+void release_free_chunks(h261g_engine *e)
 {
-    if (e->cache.empty()) return;
-    (void)hipStreamSynchronize(e->stream);
-    for (auto &kv : e->cache) hipFree(kv.second);
-    e->cache.clear();
+    bool synced = false;
+    for (size_t i = 0; i < e->chunks.size();) {
+        auto &c = e->chunks[i];
+        if (c.free.size() == 1 && c.free.begin()->first == 0 && c.free.begin()->second == c.size) {
+            if (!synced) {
+                (void)hipStreamSynchronize(e->stream);
+                (void)hipStreamSynchronize(e->in_stream);
+                synced = true;
+            }
+            hipFree(c.base);
+            e->chunks.erase(e->chunks.begin() + (long)i);
+        } else {
+            i++;
+        }
+    }
 }
 
-/* a block of at least `bytes` (and at most twice that) from the cache, else
- * a new allocation; *cap gets the block's size */
+/* a device range of at least `bytes`; *cap gets its (rounded) size */
 uint8_t *cache_get(h261g_engine *e, size_t bytes, size_t *cap)
 {
-    auto it = e->cache.lower_bound(bytes);
-    if (it != e->cache.end() && it->first <= 2 * bytes) {
-        uint8_t *ptr = it->second;
-        *cap = it->first;
-        e->cache.erase(it);
-        return ptr;
+    const size_t b = (bytes + kRangeAlign - 1) / kRangeAlign * kRangeAlign;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        for (auto &c : e->chunks)
+            for (auto it = c.free.begin(); it != c.free.end(); ++it)
+                if (it->second >= b) {
+                    const size_t off = it->first, rest = it->second - b;
+                    c.free.erase(it);
+                    if (rest) c.free.emplace(off + b, rest);
+                    *cap = b;
+                    return c.base + off;
+                }
+        if (attempt) break;
+        /* a new chunk: the standard size, or exactly this range when that
+         * is larger or the device is nearly full */
+        uint8_t *base = nullptr;
+        size_t size = b > kChunkBytes ? b : kChunkBytes;
+        if (hipMalloc((void **)&base, size) != hipSuccess) {
+            (void)hipGetLastError();
+            release_free_chunks(e);
+            size = b;
+            if (!ok(hipMalloc((void **)&base, size), "hipMalloc")) return nullptr;
+        }
+        h261g_engine::DevChunk c{base, size, {}};
+        c.free.emplace(0, size);
+        e->chunks.push_back(std::move(c));
     }
-    uint8_t *ptr = nullptr;
-    if (hipMalloc((void **)&ptr, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        cache_release(e);
-        if (!ok(hipMalloc((void **)&ptr, bytes), "hipMalloc")) return nullptr;
-    }
-    *cap = bytes;
-    return ptr;
+    return nullptr;
 }
 
+/* returns a range to its chunk, merged with free neighbours */
 void cache_put(h261g_engine *e, uint8_t *ptr, size_t cap)
 {
-    if (ptr) e->cache.emplace(cap, ptr);
+    if (!ptr) return;
+    for (auto &c : e->chunks) {
+        if (ptr < c.base || ptr >= c.base + c.size) continue;
+        size_t off = (size_t)(ptr - c.base), len = cap;
+        auto next = c.free.lower_bound(off);
+        if (next != c.free.end() && next->first == off + len) {
+            len += next->second;
+            next = c.free.erase(next);
+        }
+        if (next != c.free.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                len += prev->second;
+                c.free.erase(prev);
+            }
+        }
+        c.free.emplace(off, len);
+        return;
+    }
 }
 
 int stream_count(const h261g_engine *e) { return e->max_streams; }
@@ -536,15 +591,10 @@ int h261g_engine_destroy(h261g_engine_t *e)
     if (!e) return H261G_ERR_ARG;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->in_stream) (void)hipStreamSynchronize(e->in_stream);
-    for (auto &s : e->slots) {
+    for (auto &s : e->slots)
         if (s.gen_ev) (void)hipEventDestroy(s.gen_ev);
-        hipFree(s.input);
-        hipFree(s.recon);
-        hipFree(s.seq);
-        hipFree(s.bits);
-    }
-    for (auto &kv : e->cache) hipFree(kv.second);
-    e->cache.clear();
+    for (auto &c : e->chunks) hipFree(c.base);
+    e->chunks.clear();
     hipHostFree(e->remap_host);
     hipFree(e->remap_dev);
     if (e->remap_ev) (void)hipEventDestroy(e->remap_ev);
@@ -805,7 +855,9 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     const int64_t fsz = (int64_t)s->width * s->height * 3 / 2;
     const int64_t bytes = fsz * n_frames;
     if (e->budget && e->used + bytes > e->budget) return H261G_ERR_FULL;
-    s->seq = cache_get(e, (size_t)bytes, &s->seq_cap);
+    /* the frames, then the generator's per-frame parameters */
+    const size_t ctx_off = ((size_t)bytes + 255) & ~(size_t)255;
+    s->seq = cache_get(e, ctx_off + sizeof(content::FrameCtx) * n_frames, &s->seq_cap);
     if (!s->seq) return H261G_ERR_MEMORY;
     /* asynchronous input (a fresh stream only): generated on the input
      * stream after the compute work queued so far (a cached buffer may still
@@ -816,7 +868,8 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     if (async && (!ok(hipEventRecord(e->compute_ev, e->stream), "event") ||
                   !ok(hipStreamWaitEvent(e->in_stream, e->compute_ev, 0), "wait")))
         return H261G_ERR_DEVICE;
-    int rcode = generate_frames_device(*c, s->width, s->height, 0, n_frames, s->seq, async ? e->in_stream : e->stream);
+    int rcode = generate_frames_device(*c, s->width, s->height, 0, n_frames, s->seq,
+                                       (content::FrameCtx *)(s->seq + ctx_off), async ? e->in_stream : e->stream);
     if (rcode) {
         cache_put(e, s->seq, s->seq_cap);
         s->seq = nullptr;
@@ -1012,6 +1065,10 @@ int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *
 {
     Slot *s = get_slot(e, id);
     if (!s || !y || !cb || !cr) return H261G_ERR_ARG;
+    /* the input of the next step: generation and selection queued so far
+     * complete first */
+    if (!ok(hipStreamSynchronize(e->in_stream), "sync") || !ok(hipStreamSynchronize(e->stream), "sync"))
+        return H261G_ERR_DEVICE;
     StreamDesc d;
     if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy")) return H261G_ERR_DEVICE;
     const size_t ysz = (size_t)s->width * s->height, csz = ysz / 4;

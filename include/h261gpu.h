@@ -203,7 +203,8 @@ H261G_API int h261g_stream_read_me(h261g_engine_t *e, int32_t id, me_gpu_mb_t *m
  * buffer, it is written here on the host from the device's levels and side
  * information (the same bytes) */
 H261G_API int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int32_t cap);
-/* the current input planes (for checking device-generated content) */
+/* the current input planes, for checking device-generated content
+ * (synchronises the queued input generation and frame steps) */
 H261G_API int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
 
 /* per-kernel device time accumulated since the last reset, in ms
