@@ -32,13 +32,13 @@ namespace {
 __constant__ uint8_t c_zz[64] = H261G_ZIGZAG;
 __constant__ Vlc c_tc[kTcoeffRuns][kTcoeffLevels] = H261G_TCOEFF_TABLE;
 
-/* LDS words for one MB: its header (<= 64 bits), 6 blocks of at most
+/* LDS words of one MB's bits: its header (<= 64 bits), 6 blocks of at most
  * 8 + 63 * 20 + 2 bits, and the leading partial word */
-constexpr int kMbWords = 256;
-static_assert(kMbWords * 32 >= 31 + 64 + 6 * (8 + 63 * 20 + 2), "MB bit buffer");
+constexpr int kMbWords = 248;
+static_assert(kMbWords * 32 >= 31 + 64 + 6 * (8 + 63 * 20 + 2) + 32, "MB bit buffer");
 /* consecutive global MBs per wave: their levels arrive in one round of
  * loads (16-byte pieces of coded blocks only) */
-constexpr int kMbsPerWave = 8;
+constexpr int kMbsPerWave = 4;
 constexpr int kPieces = kMbsPerWave * 384 / 16;
 constexpr int kPieceRounds = (kPieces + 63) / 64;
 /* independent waves per workgroup: a compute unit holds a limited number of
@@ -46,6 +46,19 @@ constexpr int kPieceRounds = (kPieces + 63) / 64;
 constexpr int kWavesPerGroup = 4;
 constexpr int kTc = kTcoeffRuns * kTcoeffLevels;
 constexpr int kTcRounds = (kTc + 64 * kWavesPerGroup - 1) / (64 * kWavesPerGroup);
+/* non-zero coefficients buffered before they are coded (a block is never
+ * split between two flushes) */
+constexpr int kEntryCap = 256;
+
+/* a buffered coefficient: transmission position, level byte and flags */
+enum : uint32_t {
+    kEntFirst = 1u << 6,   /* first of its block */
+    kEntLast = 1u << 7,    /* last of its block: EOB follows */
+    kEntDc = 1u << 8,      /* intra DC: 8-bit fixed length code */
+    kEntEmpty = 1u << 9,   /* coded block without coefficients: EOB only */
+    kEntMbFirst = 1u << 12 /* first of its MB */
+};
+__device__ __forceinline__ uint32_t ent_mb(uint32_t e) { return (e >> 10) & 3u; }
 
 /* ORs the low len (<= 32) bits of code into LDS words at bit position pos,
  * first bit most significant */
@@ -62,27 +75,13 @@ __device__ __forceinline__ void lds_or(uint32_t *buf, int pos, uint32_t code, in
     }
 }
 
-/* each wave codes into its own LDS buffer, and a wave's LDS operations
+/* each wave codes into its own LDS buffers, and a wave's LDS operations
  * complete in issue order: ordering the accesses of different lanes needs
  * only a compiler barrier, not __syncthreads, whose fence would also wait
  * for every global store and atomic of the wave to complete */
 __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_wave_barrier(); }
 
 typedef __attribute__((address_space(1))) uint32_t global_u32;
-
-/* ORs code (len <= 22 bits, len 0 allowed) into LDS words at bit position
- * pos with two unconditional ds_or: a part that does not exist goes to the
- * lane's own spare word, so lanes without a code neither branch nor contend
- * for one address */
-__device__ __forceinline__ void or_code(uint32_t *buf, int pos, uint32_t code, int len, int lane)
-{
-    const int i = pos >> 5, end = (pos & 31) + len;
-    const bool split = end > 32;
-    const uint32_t hi = split ? (code >> ((end - 32) & 31)) : (code << ((32 - end) & 31));
-    const uint32_t lo = code << ((64 - end) & 31);
-    atomicOr(buf + (len ? i : kMbWords + lane), hi);
-    atomicOr(buf + (split ? i + 1 : kMbWords + lane), split ? lo : 0u);
-}
 
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
@@ -93,19 +92,92 @@ __device__ __forceinline__ int coded_blocks(const MbSide &sd)
     return (sd.flags & H261_MB_COEFF) ? ((int)(uint8_t)sd.cbp & 63) : 0;
 }
 
+/* ORs code (len <= 22 bits, len 0 allowed) into LDS words at word w, bit
+ * offset off, with two unconditional ds_or: a part that does not exist goes
+ * to the lane's own spare word, so lanes without a code neither branch nor
+ * contend for one address */
+__device__ __forceinline__ void or_code(uint32_t *lds, int w, int off, uint32_t code, int len, int spare)
+{
+    const int end = off + len;
+    const bool split = end > 32;
+    const uint32_t hi = split ? (code >> ((end - 32) & 31)) : (code << ((32 - end) & 31));
+    const uint32_t lo = code << ((64 - end) & 31);
+    atomicOr(lds + (len ? w : spare), hi);
+    atomicOr(lds + (split ? w + 1 : spare), split ? lo : 0u);
+}
+
+struct WaveLds {
+    uint32_t region[kMbsPerWave][kMbWords]; /* each MB's bits from its word phase */
+    uint32_t spare[64];
+    uint32_t ent[kEntryCap + 64];           /* buffered coefficients (+ a discard slot per lane) */
+    int32_t mb_off[kMbsPerWave];            /* bit position in the MB's region minus the running code sum */
+    uint4 lvs[kPieces];                     /* the MBs' levels, [mb][6][64] */
+};
+
+/* codes the n buffered coefficients: VLC and sign or escape, EOB after a
+ * block's last, prefix sum of the lengths, OR into the MB regions.  *sum is
+ * the running sum of code lengths over everything flushed before. */
+__device__ __forceinline__ void flush_entries(WaveLds &w, const uint32_t *tc, int n, int &sum, int lane)
+{
+    for (int base = 0; base < n; base += 64) {
+        const int j = base + lane;
+        const bool has = j < n;
+        const uint32_t e = has ? w.ent[j] : 0u;
+        const uint32_t ep = j > 0 ? w.ent[j - 1] : 0u;
+        const int pos = (int)(e & 63u);
+        const int m = (int)ent_mb(e);
+        const int prev = (e & kEntFirst) ? -1 : (int)(ep & 63u);
+        const int run = pos - prev - 1;
+        const bool dc = (e & kEntDc) != 0;
+        const int l = dc ? (int)((e >> 16) & 0xffu) : (int)(int8_t)((e >> 16) & 0xffu);
+        const int a = l < 0 ? -l : l;
+        const uint32_t sign = l < 0 ? 1u : 0u;
+        const bool in_range = run >= 0 && run < kTcoeffRuns && a < kTcoeffLevels;
+        const uint32_t te = tc[in_range ? run * kTcoeffLevels + a : 0]; /* entry 0 has length 0 */
+        const int tlen = (int)((te >> 16) & 0xffu);
+        /* escape (000001, 6-bit run, 8-bit level), else the table code and
+         * its sign, else an inter block's first coefficient +-1 at position
+         * 0 as '1s', else the intra DC as 8 bits */
+        uint32_t code = tlen ? (((te & 0xffffu) << 1) | sign)
+                             : ((1u << 14) | ((uint32_t)(run & 63) << 8) | ((uint32_t)l & 0xffu));
+        int len = tlen ? tlen + 1 : 20;
+        const bool one = pos == 0 && !dc && a == 1;
+        code = one ? (2u | sign) : code;
+        len = one ? 2 : len;
+        code = dc ? (uint32_t)l : code;
+        len = dc ? 8 : len;
+        const bool none = !has || (e & kEntEmpty);
+        code = none ? 0u : code;
+        len = none ? 0 : len;
+        /* EOB '10' after a block's last coefficient */
+        const bool last = has && (e & kEntLast);
+        code = last ? ((code << 2) | 2u) : code;
+        len = last ? len + 2 : len;
+        const int incl = fc::wave_inclusive_sum(len, lane);
+        const int at_sum = sum + incl - len; /* running sum before this code */
+        /* the MB's first coefficient fixes its offset between the running
+         * sum and its region */
+        if (has && (e & kEntMbFirst)) w.mb_off[m] += -at_sum;
+        wave_lds_order();
+        const int at = has ? w.mb_off[m] + at_sum : 0;
+        or_code(&w.region[0][0], m * kMbWords + (at >> 5), at & 31, code, len,
+                (int)((&w.spare[lane]) - &w.region[0][0]));
+        sum += __builtin_amdgcn_readlane(incl, 63);
+        wave_lds_order();
+    }
+}
+
 __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunch p)
 {
-    __shared__ uint32_t buf_all[kWavesPerGroup][kMbWords + 64]; /* + a spare word per lane */
-    __shared__ uint4 lvs_all[kWavesPerGroup][kPieces]; /* each wave's MBs' levels, [mb][6][64] */
-    __shared__ uint32_t tc[kTc];                        /* run/level table: code | len << 16 */
+    __shared__ WaveLds lds_all[kWavesPerGroup];
+    __shared__ uint32_t tc[kTc]; /* run/level table: code | len << 16 */
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int n_waves = (p.n_mb + kMbsPerWave - 1) / kMbsPerWave;
     const int n_groups = (n_waves + kWavesPerGroup - 1) / kWavesPerGroup;
     const int grp = launch_index(blockIdx.x, n_groups, p.xcd_remap);
     if (grp >= n_groups) return; /* the whole workgroup */
     const int g0 = (grp * kWavesPerGroup + wid) * kMbsPerWave;
-    uint32_t *const buf = buf_all[wid];
-    uint4 *const lvs = lvs_all[wid];
+    WaveLds &w = lds_all[wid];
 
     /* every load is issued before its value is needed: two rounds of
      * memory latency per wave (places and side information, then levels
@@ -119,7 +191,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
         tcv[i] = tcg[k < kTc ? k : 0];
     }
     const int zz = c_zz[lane];
-    /* lanes 0..7: the MBs' places, side information, streams (other lanes,
+    /* lanes 0..3: the MBs' places, side information, streams (other lanes,
      * and waves past the last MB, read a clamped index and discard it) */
     const int gl = g0 + (lane < kMbsPerWave ? lane : 0);
     const bool mine = lane < kMbsPerWave && gl < p.n_mb;
@@ -129,10 +201,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
     const MbSide sd = p.side[gc];
     const int valid = (mine && s >= 0 && h.valid) ? 1 : 0;
     const int cb = valid ? coded_blocks(sd) : 0;
-    const int hlo = (int)(uint32_t)h.code, hhi = (int)(uint32_t)(h.code >> 32);
-    const int bitpos = h.bitpos, hlen = h.len;
     const int intra = sd.type == H261_MB_TYPE_INTRA;
-    uint64_t vmask = __ballot(valid);
+    const uint64_t vmask = __ballot(valid);
     /* the coded blocks' levels and each MB's bitstream words */
     const SeqState &q = p.seq[valid ? s : 0];
     uint32_t *const words = q.bits;
@@ -154,31 +224,35 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
         const int k = threadIdx.x + 64 * kWavesPerGroup * i;
         if (k < kTc) tc[k] = tcv[i];
     }
-#pragma unroll
-    for (int i = 0; i < kMbWords / 64; i++) buf[lane + 64 * i] = 0u;
+    for (int i = lane; i < kMbsPerWave * kMbWords; i += 64) (&w.region[0][0])[i] = 0u;
+    if (lane < kMbsPerWave) w.mb_off[lane] = (h.bitpos & 31) + h.len; /* where the TCOEFF layer starts */
 #pragma unroll
     for (int j = 0; j < kPieceRounds; j++)
-        if (lane + 64 * j < kPieces) lvs[lane + 64 * j] = v[j];
+        if (lane + 64 * j < kPieces) w.lvs[lane + 64 * j] = v[j];
     __syncthreads(); /* the table from every wave */
 
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const int8_t *lvb = (const int8_t *)lvs;
-    while (vmask) {
-        const int m = __builtin_ctzll(vmask);
-        vmask &= vmask - 1;
-        const int pos0 = rl(bitpos, m), len0 = rl(hlen, m), cbm = rl(cb, m);
-        const bool intra_mb = rl(intra, m) != 0;
-        const int p0 = pos0 & 31;
-        if (lane == 0) {
-            const uint32_t lo = (uint32_t)rl(hlo, m), hi = (uint32_t)rl(hhi, m);
-            if (len0 > 32) {
-                lds_or(buf, p0, hi, len0 - 32);
-                lds_or(buf, p0 + len0 - 32, lo, 32);
-            } else {
-                lds_or(buf, p0, lo, len0);
-            }
+    /* the MB headers, lane m for MB m */
+    if (valid) {
+        const int p0 = h.bitpos & 31, len0 = h.len;
+        const uint32_t lo = (uint32_t)h.code, hi = (uint32_t)(h.code >> 32);
+        uint32_t *reg = w.region[lane];
+        if (len0 > 32) {
+            lds_or(reg, p0, hi, len0 - 32);
+            lds_or(reg, p0 + len0 - 32, lo, 32);
+        } else {
+            lds_or(reg, p0, lo, len0);
         }
-        int pos = p0 + len0;
+    }
+
+    /* the non-zero coefficients of every coded block in bitstream order,
+     * compacted: a lane per coefficient when coded */
+    const int8_t *lvb = (const int8_t *)w.lvs;
+    int n_ent = 0, sum = 0;
+    for (uint64_t mm = vmask; mm; mm &= mm - 1) {
+        const int m = __builtin_ctzll(mm);
+        const int cbm = rl(cb, m);
+        const bool intra_mb = rl(intra, m) != 0;
+        uint32_t mb_first = kEntMbFirst;
         int raws[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) raws[k] = lvb[m * 384 + k * 64 + zz];
@@ -186,61 +260,81 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
 #pragma unroll
         for (int k = 0; k < 6; k++) {
             if (!(cbm & (32 >> k))) continue;
-            /* branch-free: every lane computes every candidate code and
-             * selects (lanes of a wave diverge here on every block) */
+            if (n_ent + 64 > kEntryCap) {
+                wave_lds_order();
+                flush_entries(w, tc, n_ent, sum, lane);
+                n_ent = 0;
+            }
             const int raw = raws[k];
             const int l = dc ? (raw & 0xff) : raw; /* the intra DC is an unsigned 8-bit code */
             const bool nzl = l != 0 || dc;
-            const uint64_t bl = __ballot(nzl) & below;
-            /* run: zero positions since the previous non-zero one (an intra
-             * DC counts as non-zero; an inter block starts before position 0) */
-            const int run = lane - (bl ? 63 - __builtin_clzll(bl) : -1) - 1;
-            const int a = l < 0 ? -l : l;
-            const uint32_t sign = l < 0 ? 1u : 0u;
-            const bool in_range = run < kTcoeffRuns && a < kTcoeffLevels;
-            const uint32_t e = tc[in_range ? run * kTcoeffLevels + a : 0]; /* entry 0 has length 0 */
-            const int tlen = (int)((e >> 16) & 0xffu);
-            /* escape (000001, 6-bit run, 8-bit level), else the table code
-             * and its sign, else an inter block's first +-1 as '1s', else
-             * the intra DC as 8 bits */
-            uint32_t code = tlen ? (((e & 0xffffu) << 1) | sign) : ((1u << 14) | ((uint32_t)run << 8) | ((uint32_t)l & 0xffu));
-            int len = tlen ? tlen + 1 : 20;
-            const bool first = lane == 0 && a == 1;
-            code = first ? (2u | sign) : code;
-            len = first ? 2 : len;
-            code = dc ? (uint32_t)l : code;
-            len = dc ? 8 : len;
-            code = nzl ? code : 0u;
-            len = nzl ? len : 0;
-            /* EOB '10' closes the block: appended to lane 63's code */
-            code = lane == 63 ? ((code << 2) | 2u) : code;
-            len = lane == 63 ? len + 2 : len;
-            const int incl = fc::wave_inclusive_sum(len, lane);
-            or_code(buf, pos + incl - len, code, len, lane);
-            pos += __builtin_amdgcn_readlane(incl, 63);
-        }
-        wave_lds_order();
-        /* words inside the MB are its own: plain stores; the first and last
-         * are shared with neighbours: atomic ORs.  The buffer is left zero. */
-        const int n = (pos + 31) >> 5;
-        const int w0 = pos0 >> 5;
-        /* a global-address-space pointer: flat stores would also count
-         * against the LDS wait counter and make every LDS wait in the loop
-         * wait for them */
-        global_u32 *dst = (global_u32 *)(((uint64_t)(uint32_t)rl((int)(uint32_t)(uint64_t)words, m)) |
-                                         ((uint64_t)(uint32_t)rl((int)(uint32_t)((uint64_t)words >> 32), m) << 32));
-        const bool fits = w0 + n <= rl(cap, m);
-        for (int i = lane; i < n; i += 64) {
-            const uint32_t v = __builtin_bswap32(buf[i]);
-            buf[i] = 0u;
-            if (!fits) continue;
-            if (i == 0 || i == n - 1) {
-                if (v) __hip_atomic_fetch_or(dst + w0 + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t nz = __ballot(nzl);
+            const int cnt = __builtin_popcountll(nz);
+            const uint32_t tag = ((uint32_t)m << 10) | mb_first;
+            if (cnt == 0) {
+                if (lane == 0) w.ent[n_ent] = tag | kEntFirst | kEntLast | kEntEmpty;
+                n_ent += 1;
             } else {
-                dst[w0 + i] = v;
+                const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(nz >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nz, 0u));
+                const uint32_t e = (uint32_t)lane | tag | (rank == 0 ? kEntFirst : 0u) |
+                                   (rank == cnt - 1 ? kEntLast : 0u) | (dc ? kEntDc : 0u) |
+                                   (((uint32_t)l & 0xffu) << 16);
+                /* the first entry alone carries the MB-first flag */
+                w.ent[nzl ? n_ent + rank : kEntryCap + lane] = rank == 0 ? e : (e & ~kEntMbFirst);
+                n_ent += cnt;
             }
+            mb_first = 0;
         }
+    }
+    if (n_ent) {
         wave_lds_order();
+        flush_entries(w, tc, n_ent, sum, lane);
+    }
+    wave_lds_order();
+
+    /* the MBs' words, all MBs at once: lane i stores word i of the MBs'
+     * words laid end to end.  Inside an MB a word is its own (plain store);
+     * its first and last are shared with neighbours (atomic OR).  An MB whose
+     * frame outgrew the buffer stores nothing (written on the host). */
+    int nw = 0;
+    if (valid) {
+        const int n = ((h.bitpos & 31) + h.bits + 31) >> 5;
+        nw = (h.bitpos >> 5) + n <= cap ? n : 0;
+    }
+    const uint64_t dst_mb = (uint64_t)(words + (h.bitpos >> 5));
+    int n_of[kMbsPerWave], o_of[kMbsPerWave];
+    uint64_t d_of[kMbsPerWave];
+    int total = 0;
+#pragma unroll
+    for (int m = 0; m < kMbsPerWave; m++) {
+        n_of[m] = rl(nw, m);
+        o_of[m] = total;
+        total += n_of[m];
+        d_of[m] = (uint64_t)(uint32_t)rl((int)(uint32_t)dst_mb, m) |
+                  ((uint64_t)(uint32_t)rl((int)(uint32_t)(dst_mb >> 32), m) << 32);
+    }
+    for (int i = lane; i < total; i += 64) {
+        int m = 0;
+#pragma unroll
+        for (int k = 1; k < kMbsPerWave; k++) m += i >= o_of[k] ? 1 : 0;
+        int n = n_of[0], o = o_of[0];
+        uint64_t d = d_of[0];
+#pragma unroll
+        for (int k = 1; k < kMbsPerWave; k++) {
+            n = m == k ? n_of[k] : n;
+            o = m == k ? o_of[k] : o;
+            d = m == k ? d_of[k] : d;
+        }
+        const int k = i - o;
+        /* a global-address-space pointer: flat stores would also count
+         * against the LDS wait counter */
+        global_u32 *dst = (global_u32 *)d;
+        const uint32_t val = __builtin_bswap32(w.region[m][k]);
+        if (k == 0 || k == n - 1) {
+            if (val) __hip_atomic_fetch_or(dst + k, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            dst[k] = val;
+        }
     }
 }
 

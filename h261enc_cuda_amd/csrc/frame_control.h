@@ -287,8 +287,9 @@ __device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide
             MbHdr h;
             h.code = hc;
             h.bitpos = gob_pos + kGobHeaderBits + incl - b;
-            h.len = (int16_t)hl;
+            h.len = (uint8_t)hl;
             h.valid = 1;
+            h.bits = (uint16_t)b;
             hdr[d.mb_base + (gy + lane / 11) * d.mb_w + gx + lane % 11] = h;
         }
         pos = gob_pos + kGobHeaderBits + __builtin_amdgcn_readlane(incl, 63);

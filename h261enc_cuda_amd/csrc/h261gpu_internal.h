@@ -65,8 +65,9 @@ struct SeqState {
 struct MbHdr {
     uint64_t code;  /* MBA, MTYPE, MQUANT, MVD and CBP codes, first bit most significant */
     int32_t bitpos; /* bit offset of the MBA code in the frame */
-    int16_t len;    /* bits in code */
-    int16_t valid;  /* 1: written by the next writer launch, which clears it */
+    uint8_t len;    /* bits in code */
+    uint8_t valid;  /* 1: written by the next writer launch, which clears it */
+    uint16_t bits;  /* all bits of the MB: header codes and TCOEFF layer */
 };
 static_assert(sizeof(MbHdr) == 16, "MbHdr layout");
 
