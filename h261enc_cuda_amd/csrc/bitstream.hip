@@ -41,9 +41,10 @@ static_assert(kMbWords * 32 >= 31 + 64 + 6 * (8 + 63 * 20 + 2) + 32, "MB bit buf
 constexpr int kMbsPerWave = 4;
 constexpr int kPieces = kMbsPerWave * 384 / 16;
 constexpr int kPieceRounds = (kPieces + 63) / 64;
-/* independent waves per workgroup: a compute unit holds a limited number of
- * workgroups, so one-wave workgroups would cap the waves resident per SIMD */
-constexpr int kWavesPerGroup = 4;
+/* independent waves per workgroup (each with its own LDS buffers; measured
+ * at 512 streams: 1 wave 38 us, 2 or 4 waves 40 us, 8 waves 54 us per
+ * launch at full load) */
+constexpr int kWavesPerGroup = 1;
 constexpr int kTc = kTcoeffRuns * kTcoeffLevels;
 constexpr int kTcRounds = (kTc + 64 * kWavesPerGroup - 1) / (64 * kWavesPerGroup);
 /* non-zero coefficients buffered before they are coded (a block is never
@@ -247,15 +248,20 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
     /* the non-zero coefficients of every coded block in bitstream order,
      * compacted: a lane per coefficient when coded */
     const int8_t *lvb = (const int8_t *)w.lvs;
+    /* every block's level at this lane's transmission position, read once */
+    int raw_all[kMbsPerWave][6];
+#pragma unroll
+    for (int m = 0; m < kMbsPerWave; m++)
+#pragma unroll
+        for (int k = 0; k < 6; k++) raw_all[m][k] = lvb[m * 384 + k * 64 + zz];
     int n_ent = 0, sum = 0;
-    for (uint64_t mm = vmask; mm; mm &= mm - 1) {
-        const int m = __builtin_ctzll(mm);
+#pragma unroll
+    for (int m = 0; m < kMbsPerWave; m++) {
+        if (!((vmask >> m) & 1)) continue;
         const int cbm = rl(cb, m);
         const bool intra_mb = rl(intra, m) != 0;
         uint32_t mb_first = kEntMbFirst;
-        int raws[6];
-#pragma unroll
-        for (int k = 0; k < 6; k++) raws[k] = lvb[m * 384 + k * 64 + zz];
+        const int(&raws)[6] = raw_all[m];
         const bool dc = intra_mb && lane == 0;
 #pragma unroll
         for (int k = 0; k < 6; k++) {
