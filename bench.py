@@ -52,8 +52,10 @@ MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 3
 # streams active, 126,720 MBs per dispatch), each scaled by what the same
 # counter reads for the 256 MiB device copy of --copy-probe, divided by the
 # dispatch's MB count (profiles/batched_512_streams_hbm_traffic.md: motion
-# estimation 508.1 MB read + 16.3 MB written, the coder 98.8 + 129.4 MB).
-TRAFFIC_PER_MB = {"me_kernel": (508.13e6 + 16.29e6) / 126720, "mb_kernel": (98.77e6 + 129.38e6) / 126720}
+# estimation 508.6 MB read + 16.3 MB written, the coder 99.0 + 133.0 MB, the
+# bitstream writer 12.7 + 6.7 MB).
+TRAFFIC_PER_MB = {"me_kernel": (508.65e6 + 16.29e6) / 126720, "mb_kernel": (99.05e6 + 132.96e6) / 126720,
+                  "bitstream_kernel": (12.72e6 + 6.68e6) / 126720}
 
 
 def traffic_per_launch(kernel, mbs_per_launch):
