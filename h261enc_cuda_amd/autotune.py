@@ -1,46 +1,88 @@
 """Autotuner for the engine's launch shape.
 
-The macroblock coder codes `mbs_per_wave` macroblocks in sequence per wave
-(statistics for the denoise update are reduced in LDS and flushed to HBM
-once per wave, so more MBs per wave means fewer global atomics but fewer,
-longer waves).  The tuner times each candidate on a sample of the target
-workload -- the same mix of stream sizes, all streams active, inter frames
--- with HIP events around the kernel launches, and returns the fastest.
+Two knobs change how the frame step's work is laid out on the device, not
+what it computes:
+
+* `mbs_per_wave` -- macroblocks one wave of the coder codes in sequence
+  (0 = chosen from the batch size).  More MBs per wave means fewer waves and
+  fewer flushes of the denoise statistics, but longer waves.
+* `xcd_remap` -- bit 0 motion estimation, bit 1 coder, bit 2 bitstream
+  writer: whether each XCD gets a contiguous run of macroblocks (neighbours
+  share that XCD's L2) or workgroup i codes macroblock i.
+
+The tuner times candidates on a sample of the target workload -- the same
+mix of stream sizes, all streams active, inter frames -- with HIP events
+around every launch of the step, and keeps the fastest: first over
+`mbs_per_wave` at the default remap, then over the remap at the best
+`mbs_per_wave` (coordinate descent; the two knobs act on mostly different
+kernels).  One engine serves every trial; each trial first runs one
+untimed step with its settings.
 """
 from __future__ import annotations
 
-from typing import Dict, Iterable, List, Tuple
+from typing import Dict, Iterable, List, Sequence, Tuple
 
+from . import _native as nat
 from .encoder import Engine
 
-_CACHE: Dict[Tuple, int] = {}
+_CACHE: Dict[Tuple, dict] = {}
 
 
-def tune_mbs_per_wave(n_cif: int, n_qcif: int, candidates: Iterable[int] = (1, 2, 4, 8), steps: int = 6,
-                      seed: int = 99) -> Tuple[int, List[dict]]:
-    """Returns (best mbs_per_wave, per-candidate timings)."""
-    key = (n_cif, n_qcif, tuple(candidates))
-    results = []
-    for mpw in candidates:
-        eng = Engine(max_streams=n_cif + n_qcif, mbs_per_wave=mpw)
-        for i in range(n_cif + n_qcif):
-            size = (352, 288) if i < n_cif else (176, 144)
-            sid = eng.open(*size)
-            eng.generate_input(sid, seed + i, steps + 2, sigma=2, motion=15)
-        eng.step()  # intra frame
+def _step_ms(t: dict) -> float:
+    n = max(1, t["steps"])
+    return (t["me_ms"] + t["mb_ms"] + t["other_ms"] + t["bits_ms"]) / n
+
+
+def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 2, 4),
+         xcd_remap: Iterable[int] = tuple(range(8)), steps: int = 4, seed: int = 99) -> dict:
+    """sizes: (width, height) of every stream of the sample.  Returns
+    {"mbs_per_wave": best, "xcd_remap": best, "step_ms": its time per step,
+    "trials": [{"mbs_per_wave", "xcd_remap", "step_ms", kernel times}]}."""
+    mpw_c, remap_c = list(mbs_per_wave), list(xcd_remap)
+    key = (tuple(sizes), tuple(mpw_c), tuple(remap_c), steps)
+    if key in _CACHE:
+        return _CACHE[key]
+    frames = 2 + (steps + 1) * (len(mpw_c) + len(remap_c))
+    eng = Engine(max_streams=len(sizes))
+    trials: List[dict] = []
+    try:
+        for i, (w, h) in enumerate(sizes):
+            sid = eng.open(w, h)
+            eng.generate_input(sid, seed + i, frames, sigma=2, motion=15)
+        eng.step()  # the intra frame
+        eng.step()
         eng.sync()
-        eng.profile(True)
-        for _ in range(steps):
+
+        def trial(mpw: int, remap: int) -> float:
+            eng.config(nat.OPT_MBS_PER_WAVE, mpw)
+            eng.config(nat.OPT_XCD_REMAP, remap)
             eng.step()
-        eng.sync()
-        t = eng.kernel_times()
+            eng.sync()
+            eng.profile(True)
+            for _ in range(steps):
+                eng.step()
+            t = eng.kernel_times()
+            eng.profile(False)
+            ms = _step_ms(t)
+            n = max(1, t["steps"])
+            trials.append({"mbs_per_wave": mpw, "xcd_remap": remap, "step_ms": round(ms, 4),
+                           "me_ms": round(t["me_ms"] / n, 4), "mb_ms": round(t["mb_ms"] / n, 4),
+                           "bits_ms": round(t["bits_ms"] / n, 4)})
+            return ms
+
+        default_remap = 2
+        best_mpw = min(mpw_c, key=lambda m: trial(m, default_remap))
+        best_remap = min(remap_c, key=lambda r: trial(best_mpw, r))
+    finally:
         eng.destroy()
-        results.append({"mbs_per_wave": mpw, "mb_ms": t["mb_ms"] / max(1, t["steps"]),
-                        "me_ms": t["me_ms"] / max(1, t["steps"])})
-    best = min(results, key=lambda r: r["mb_ms"])["mbs_per_wave"]
-    _CACHE[key] = best
-    return best, results
+    best = min((t for t in trials if t["mbs_per_wave"] == best_mpw and t["xcd_remap"] == best_remap),
+               key=lambda t: t["step_ms"])
+    out = {"mbs_per_wave": best_mpw, "xcd_remap": best_remap, "step_ms": best["step_ms"], "trials": trials}
+    _CACHE[key] = out
+    return out
 
 
-def cached(n_cif: int, n_qcif: int, candidates: Iterable[int] = (1, 2, 4, 8)):
-    return _CACHE.get((n_cif, n_qcif, tuple(candidates)))
+def apply(eng: Engine, tuned: dict) -> None:
+    """Sets a tuned launch shape on an engine."""
+    eng.config(nat.OPT_MBS_PER_WAVE, tuned["mbs_per_wave"])
+    eng.config(nat.OPT_XCD_REMAP, tuned["xcd_remap"])

@@ -157,7 +157,8 @@ struct h261g_engine {
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_variant = kMeTwoStage;
     int fused_tails = 1;
-    /* bit 0: motion estimation, bit 1: macroblock coder.  Measured at 512
+    /* bit 0: motion estimation, bit 1: macroblock coder, bit 2: bitstream
+     * writer (measured: the writer is faster without).  Measured at 512
      * streams: the remap cuts the coder's L2 misses ~9x and its time ~4% (at
      * 4 MBs per wave); it cuts motion estimation's misses ~7.6x but not its
      * time (that kernel is VALU/LDS bound, its re-reads are absorbed below
@@ -447,7 +448,7 @@ int enqueue_step(h261g_engine *e, bool timed)
         wl.hdr = hdr;
         wl.side = e->side;
         wl.levels = e->levels;
-        wl.xcd_remap = (e->xcd_remap >> 1) & 1;
+        wl.xcd_remap = (e->xcd_remap >> 2) & 1;
         if (launch_bitstream(wl, st, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr)) return H261G_ERR_DEVICE;
     } else if (timed) {
         (void)hipEventRecord(ev[6], st);
@@ -512,7 +513,7 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         e->fused_tails = value;
         break;
     case H261G_OPT_XCD_REMAP:
-        if (value < 0 || value > 3) return H261G_ERR_ARG;
+        if (value < 0 || value > 7) return H261G_ERR_ARG;
         e->xcd_remap = value;
         break;
     case H261G_OPT_ASYNC_INPUT:

@@ -123,7 +123,8 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                     (0: chosen per launch from the batch size) */
 #define H261G_OPT_FUSED_TAILS 3  /* 1 (default): per-stream frame control runs in the tails of the
                                     two frame-step kernels; 0: as four separate launches */
-#define H261G_OPT_XCD_REMAP 4    /* bit 0: motion estimation, bit 1: macroblock coder; a set bit
+#define H261G_OPT_XCD_REMAP 4    /* bit 0: motion estimation, bit 1: macroblock coder, bit 2:
+                                    bitstream writer; a set bit
                                     gives each XCD a contiguous run of macroblocks (neighbours
                                     share its L2), a clear bit has workgroup i code macroblock i.
                                     Default 2. */
