@@ -225,13 +225,14 @@ def hard_motion(Engine, steps=120, mbs_per_wave=0):
 def serving(rate=2000.0, duration=0.5):
     """Poisson arrivals (60% CIF / 40% QCIF, lognormal lengths, 64-1024
     kbit/s, local-motion content), admission by HBM budget, continuous
-    batching"""
+    batching; every step's bitstreams delivered to host memory"""
     from h261enc_cuda_amd.serving import ServingLoop, poisson_requests
     reqs = poisson_requests(rate=rate, duration=duration)
     loop = ServingLoop(hbm_budget=96 << 30, max_streams=4096)
-    st = loop.run(reqs)
+    st = loop.run(reqs, deliver=True)
     loop.close()
     return {"rate_req_s": rate, "arrival_window_s": duration, "requests": st.requests, "frames": st.frames,
+            "latency_ends": "bitstream in host memory", "bitstream_mb_delivered": round(st.delivered_bytes / 1e6, 2),
             "seconds": round(st.seconds, 3), "frames_per_s": round(st.frames_per_s, 1),
             "cif_frames_per_s": round(st.cif_frames_per_s, 1),
             "frame_latency_ms_p50": round(st.frame_latency_ms_p50, 3),

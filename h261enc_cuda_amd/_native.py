@@ -50,7 +50,8 @@ EXPORTS = [
     "h261g_engine_sync", "h261g_stream_info", "h261g_stream_read_macroblocks", "h261g_stream_read_recon",
     "h261g_stream_read_me", "h261g_stream_read_bitstream", "h261g_stream_read_input",
     "h261g_engine_profile", "h261g_engine_kernel_times", "h261g_content_frame_host", "h261g_write_picture",
-    "h261g_engine_config", "h261g_device_info", "h261g_stream_input_pending",
+    "h261g_engine_config", "h261g_device_info", "h261g_stream_input_pending", "h261g_engine_fetch_bitstreams",
+    "h261g_engine_fetched",
 ]
 OPT_ME_VARIANT = 1
 OPT_MBS_PER_WAVE = 2
@@ -119,6 +120,8 @@ def load() -> ctypes.CDLL:
         "h261g_stream_read_recon": (ctypes.c_int, [P, i32, P, P, P]),
         "h261g_stream_read_me": (ctypes.c_int, [P, i32, P]),
         "h261g_stream_read_bitstream": (ctypes.c_int, [P, i32, P, i32]),
+        "h261g_engine_fetch_bitstreams": (ctypes.c_int, [P, P, i32]),
+        "h261g_engine_fetched": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]),
         "h261g_stream_read_input": (ctypes.c_int, [P, i32, P, P, P]),
         "h261g_engine_profile": (ctypes.c_int, [P, i32]),
         "h261g_engine_kernel_times": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double),

@@ -344,6 +344,58 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
     }
 }
 
+/* sizes and places of fetched bitstreams, one workgroup: stream ids[i]'s
+ * last frame is on the device when it fit the stream's buffer.  Places are
+ * 4-byte aligned, a running scan over chunks of the list; a frame that is
+ * not on the device, or whose place ends past the host buffer's capacity,
+ * gets size -1 (written on the host instead).  meta (host memory): sizes[n],
+ * then offsets[n + 1], offsets[n] being the bytes all frames would take. */
+/* one wave: it needs only one wave slot, so it is dispatched at once even
+ * while other work fills the device */
+__global__ __launch_bounds__(64) void fetch_plan_kernel(const int32_t *ids, int n, const SeqState *seq,
+                                                        const RcState *rc, int32_t *dev_size, int32_t *dev_off,
+                                                        int32_t *meta, int64_t data_cap)
+{
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        int size = -1;
+        if (i < n) {
+            const int s = ids[i];
+            const SeqState &q = seq[s];
+            const RcState &r = rc[s];
+            if (q.bits && r.frame_bits > 0 && (r.frame_bits + 31) / 32 <= q.bits_words) size = r.frame_bytes;
+        }
+        const int span = size > 0 ? (size + 3) & ~3 : 0;
+        const int incl = fc::wave_inclusive_sum(span, lane);
+        const int off = carry + incl - span;
+        if ((int64_t)off + span > data_cap) size = -1;
+        if (i < n) {
+            dev_size[i] = size;
+            dev_off[i] = off;
+            meta[i] = size;
+            meta[n + i] = off;
+        }
+        carry += __builtin_amdgcn_readlane(incl, 63);
+    }
+    if (lane == 0) meta[2 * n] = carry;
+}
+
+/* one workgroup per listed stream: its frame's words into host memory */
+__global__ __launch_bounds__(256) void fetch_copy_kernel(const int32_t *ids, const SeqState *seq,
+                                                         const int32_t *dev_size, const int32_t *dev_off,
+                                                         uint32_t *host_data)
+{
+    const int i = blockIdx.x;
+    const int size = dev_size[i];
+    if (size <= 0) return;
+    const uint32_t *src = seq[ids[i]].bits;
+    uint32_t *dst = host_data + dev_off[i] / 4;
+    const int words = (size + 3) / 4;
+    for (int k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
+}
+
 }  // namespace
 
 int launch_bitstream(const BitLaunch &p, void *stream, void *ev_start, void *ev_stop)
@@ -361,6 +413,18 @@ int launch_bitstream(const BitLaunch &p, void *stream, void *ev_start, void *ev_
                               (hipEvent_t)ev_stop, 0, p);
     else
         hipLaunchKernelGGL(bitstream_kernel, grid, dim3(64 * kWavesPerGroup), 0, (hipStream_t)stream, p);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fetch(const int32_t *ids, int n, const SeqState *seq, const RcState *rc, int32_t *dev_size,
+                 int32_t *dev_off, int32_t *meta, uint8_t *host_data, int64_t data_cap, void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(fetch_plan_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ids, n, seq, rc, dev_size, dev_off,
+                       meta, data_cap);
+    if (hipGetLastError() != hipSuccess) return -1;
+    hipLaunchKernelGGL(fetch_copy_kernel, dim3(n), dim3(256), 0, (hipStream_t)stream, ids, seq, dev_size, dev_off,
+                       (uint32_t *)host_data);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

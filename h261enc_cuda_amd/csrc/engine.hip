@@ -185,6 +185,16 @@ struct h261g_engine {
     MbHdr *hdr = nullptr;     /* [mb]: bitstream places, frame tail -> writer (all invalid between steps) */
     std::vector<int> free_cif, free_qcif; /* MB ranges of closed streams */
     h261_macroblock_t *mbstruct = nullptr; /* scratch for read_macroblocks */
+    /* bulk bitstream fetch: ids, sizes/offsets and bytes in pinned host
+     * memory (the gather kernels write the host memory directly), device
+     * scratch, and the event of the fetch in flight */
+    int32_t *fetch_ids = nullptr, *fetch_meta = nullptr, *fetch_dev = nullptr;
+    uint8_t *fetch_data = nullptr;
+    int fetch_cap = 0;
+    size_t fetch_data_cap = 0;
+    int fetch_n = 0;
+    bool fetch_pending = false;
+    hipEvent_t fetch_ev = nullptr;
     int mbstruct_cap = 0;
     /* graph of one step */
     hipGraph_t graph = nullptr;
@@ -612,6 +622,11 @@ int h261g_engine_destroy(h261g_engine_t *e)
     hipFree(e->levels);
     hipFree(e->hdr);
     hipHostFree(e->mbstruct);
+    hipHostFree(e->fetch_ids);
+    hipHostFree(e->fetch_meta);
+    hipHostFree(e->fetch_data);
+    hipFree(e->fetch_dev);
+    if (e->fetch_ev) (void)hipEventDestroy(e->fetch_ev);
     for (int r = 0; r < h261g_engine::kEvRing; r++)
         for (int i = 0; i < 8; i++)
             if (e->ev[r][i]) (void)hipEventDestroy(e->ev[r][i]);
@@ -1108,6 +1123,75 @@ int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int
     h261g_frame_info_t info;
     if ((rcode = h261g_stream_info(e, id, &info))) return rcode;
     return h261g_write_picture(e->mbstruct, s->width, s->height, info.frame_num, info.quant, out, cap);
+}
+
+int h261g_engine_fetch_bitstreams(h261g_engine_t *e, const int32_t *ids, int32_t n)
+{
+    if (!e || n < 0 || (n && !ids)) return H261G_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (!get_slot(e, ids[i])) return H261G_ERR_ARG;
+    /* the previous fetch's buffers are rewritten below */
+    if (e->fetch_pending && !ok(hipEventSynchronize(e->fetch_ev), "event")) return H261G_ERR_DEVICE;
+    e->fetch_pending = false;
+    /* the host buffer grows to twice what the previous fetch needed; frames
+     * that do not fit are written on the host when collected */
+    size_t data = e->fetch_data_cap ? e->fetch_data_cap : ((size_t)4 << 20);
+    if (e->fetch_n && e->fetch_meta && (size_t)e->fetch_meta[2 * e->fetch_n] > e->fetch_data_cap)
+        data = 2 * (size_t)e->fetch_meta[2 * e->fetch_n];
+    if (!e->fetch_ev && !ok(hipEventCreateWithFlags(&e->fetch_ev, hipEventDisableTiming), "event"))
+        return H261G_ERR_DEVICE;
+    if (n > e->fetch_cap) {
+        if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
+        hipHostFree(e->fetch_ids);
+        hipHostFree(e->fetch_meta);
+        hipFree(e->fetch_dev);
+        e->fetch_ids = e->fetch_meta = e->fetch_dev = nullptr;
+        e->fetch_cap = 0;
+        e->fetch_n = 0;
+        const int cap = n > e->max_streams ? n + n / 2 : e->max_streams;
+        if (!ok(hipHostMalloc((void **)&e->fetch_ids, sizeof(int32_t) * cap, hipHostMallocDefault), "hipHostMalloc") ||
+            !ok(hipHostMalloc((void **)&e->fetch_meta, sizeof(int32_t) * (2 * cap + 1), hipHostMallocNonCoherent),
+                "hipHostMalloc") ||
+            !ok(hipMalloc((void **)&e->fetch_dev, sizeof(int32_t) * 3 * cap), "hipMalloc"))
+            return H261G_ERR_MEMORY;
+        e->fetch_cap = cap;
+    }
+    if (data > e->fetch_data_cap) {
+        /* no fetch is in flight (waited above), so the old buffer is free */
+        hipHostFree(e->fetch_data);
+        e->fetch_data = nullptr;
+        e->fetch_data_cap = 0;
+        /* written by the gather kernels only, read by the host after the
+         * fetch's event: non-coherent, so the device writes it through its
+         * caches in whole lines rather than word by word across the bus */
+        if (!ok(hipHostMalloc((void **)&e->fetch_data, data, hipHostMallocNonCoherent), "hipHostMalloc"))
+            return H261G_ERR_MEMORY;
+        e->fetch_data_cap = data;
+    }
+    memcpy(e->fetch_ids, ids, sizeof(int32_t) * n);
+    e->fetch_n = n;
+    /* the ids to device memory by DMA; scratch: ids, sizes, places */
+    int32_t *dev_ids = e->fetch_dev + 2 * e->fetch_cap;
+    if (n && (!ok(hipMemcpyAsync(dev_ids, e->fetch_ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream),
+                  "hipMemcpy") ||
+              launch_fetch(dev_ids, n, e->seq, e->rc, e->fetch_dev, e->fetch_dev + e->fetch_cap, e->fetch_meta,
+                           e->fetch_data, (int64_t)e->fetch_data_cap, e->stream)))
+        return H261G_ERR_DEVICE;
+    if (!ok(hipEventRecord(e->fetch_ev, e->stream), "event")) return H261G_ERR_DEVICE;
+    e->fetch_pending = true;
+    return H261G_OK;
+}
+
+int h261g_engine_fetched(h261g_engine_t *e, const uint8_t **data, const int32_t **sizes, const int32_t **offsets)
+{
+    if (!e || !data || !sizes || !offsets) return H261G_ERR_ARG;
+    if (!e->fetch_ev || (!e->fetch_pending && !e->fetch_n)) return H261G_ERR_STATE;
+    if (e->fetch_pending && !ok(hipEventSynchronize(e->fetch_ev), "event")) return H261G_ERR_DEVICE;
+    e->fetch_pending = false;
+    *data = e->fetch_data;
+    *sizes = e->fetch_meta;
+    *offsets = e->fetch_meta + e->fetch_n;
+    return e->fetch_n;
 }
 
 int h261g_engine_profile(h261g_engine_t *e, int32_t enable)

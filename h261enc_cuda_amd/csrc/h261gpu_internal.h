@@ -220,6 +220,14 @@ struct BitLaunch {
 };
 int launch_bitstream(const BitLaunch &p, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
 
+/* the last frame's bitstream of streams ids[0..n) (ids readable by the
+ * device) into host memory: sizes and 4-byte-aligned places into meta
+ * (sizes[n], offsets[n + 1]; size -1: not on the device or past data_cap
+ * bytes), the bytes into host_data; dev_size / dev_off: device scratch of n
+ * entries each */
+int launch_fetch(const int32_t *ids, int n, const SeqState *seq, const RcState *rc, int32_t *dev_size,
+                 int32_t *dev_off, int32_t *meta, uint8_t *host_data, int64_t data_cap, void *stream);
+
 }  // namespace h261g
 
 #endif

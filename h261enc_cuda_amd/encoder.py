@@ -9,7 +9,7 @@ write_picture the H.261 writer over a frame's macroblock structs
 from __future__ import annotations
 
 import ctypes
-from typing import Optional, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -146,6 +146,41 @@ class Engine:
         out = np.zeros(cap, np.uint8)
         n = check(self.lib.h261g_stream_read_bitstream(self.h, sid, ptr(out), cap), "read_bitstream")
         return out[:n].tobytes()
+
+    def fetch_bitstreams(self, sids) -> None:
+        """Queues delivery of the listed streams' last-frame bitstreams to
+        host memory (returns at once; `fetched` collects them)."""
+        ids = np.ascontiguousarray(np.asarray(sids, dtype=np.int32))
+        check(self.lib.h261g_engine_fetch_bitstreams(self.h, ptr(ids), len(ids)), "fetch_bitstreams")
+        self._fetch_ids = ids.copy()
+
+    def fetch_wait(self) -> int:
+        """Waits for the last fetch_bitstreams; returns the bytes delivered
+        (the bitstreams stay in the engine's pinned buffer)."""
+        data, sizes, offs = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        n = check(self.lib.h261g_engine_fetched(self.h, ctypes.byref(data), ctypes.byref(sizes), ctypes.byref(offs)),
+                  "fetched")
+        return int(ctypes.cast(offs, ctypes.POINTER(ctypes.c_int32))[n]) if n else 0
+
+    def fetched(self) -> List[bytes]:
+        """Waits for the last fetch_bitstreams; one bytes object per listed
+        stream, in order (a frame that outgrew its device buffer is written
+        on the host here)."""
+        data, sizes, offs = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        n = check(self.lib.h261g_engine_fetched(self.h, ctypes.byref(data), ctypes.byref(sizes), ctypes.byref(offs)),
+                  "fetched")
+        if n == 0:
+            return []
+        sz = np.ctypeslib.as_array(ctypes.cast(sizes, ctypes.POINTER(ctypes.c_int32)), (n,)).copy()
+        of = np.ctypeslib.as_array(ctypes.cast(offs, ctypes.POINTER(ctypes.c_int32)), (n + 1,)).copy()
+        buf = ctypes.string_at(data, int(of[n])) if of[n] else b""
+        out = []
+        for i in range(n):
+            if sz[i] >= 0:
+                out.append(buf[of[i]:of[i] + sz[i]])
+            else:
+                out.append(self.bitstream(int(self._fetch_ids[i])))
+        return out
 
     def profile(self, enable: bool) -> None:
         check(self.lib.h261g_engine_profile(self.h, int(enable)), "profile")

@@ -9,10 +9,13 @@ streams are closed and their memory and macroblock range reused.
 
 Per-frame latency = time from the frame becoming eligible (its input
 available and its previous frame done) to the end of the step that coded it
-(levels, side information and the exact frame size on the device).  A
-request's input is a synthetic sequence generated on the device; the time
-from its arrival until that input is available is reported separately as
-the input wait, and is part of the request latency (arrival to last frame).
+(levels, side information, reconstruction and the H.261 bitstream on the
+device) -- with `deliver`, to the bitstream's arrival in host memory (every
+step's bitstreams are gathered into pinned host memory by one queued
+operation).  A request's input is a synthetic sequence generated on the
+device; the time from its arrival until that input is available is reported
+separately as the input wait, and is part of the request latency (arrival to
+last frame).
 """
 from __future__ import annotations
 
@@ -73,6 +76,7 @@ class ServingStats:
     input_wait_ms_p99: float
     max_concurrent: int
     steps: int
+    delivered_bytes: int = 0
 
     @property
     def frames_per_s(self) -> float:
@@ -91,9 +95,11 @@ class ServingLoop:
         # the first step after they are ready
         self.eng.config(nat.OPT_ASYNC_INPUT, int(async_input))
 
-    def run(self, requests: List[Request], collect: bool = False, realtime: bool = True):
-        """Serves the requests.  collect=True also returns every request's
-        per-frame bitstreams (slow: the host writer runs after each step)."""
+    def run(self, requests: List[Request], collect: bool = False, realtime: bool = True, deliver: bool = False):
+        """Serves the requests.  deliver=True brings every step's bitstreams
+        to host memory before the step counts as done; collect=True (implies
+        deliver) also returns every request's per-frame bitstreams."""
+        deliver = deliver or collect
         eng = self.eng
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting: collections.deque = collections.deque()
@@ -111,7 +117,7 @@ class ServingLoop:
         req_lat: List[float] = []
         input_wait: List[float] = []
         t_start = time.perf_counter()
-        frames = mbs = steps = max_conc = 0
+        frames = mbs = steps = max_conc = delivered = 0
         while pending or waiting or len(sids):
             now = time.perf_counter() - t_start
             while pending and (pending[0].arrival <= now or not realtime):
@@ -157,18 +163,23 @@ class ServingLoop:
                     started[k] = True
                     eligible[k] = t_call
                     input_wait.append((t_call - arrival[k]) * 1e3)
+            run = started
+            if deliver:
+                eng.fetch_bitstreams(sids[run])
             eng.sync()
+            if deliver:
+                if collect:
+                    for ri, bits in zip(req_idx[run].tolist(), eng.fetched()):
+                        outputs[ri].append(bits)
+                else:
+                    delivered += eng.fetch_wait()
             steps += 1
             t_end = time.perf_counter() - t_start
-            run = started
             frame_lat.append((t_end - eligible[run]) * 1e3)
             eligible[run] = t_end
             left[run] -= 1
             frames += int(run.sum())
             mbs += int(nmb[run].sum())
-            if collect:
-                for sid, ri in zip(sids[run].tolist(), req_idx[run].tolist()):
-                    outputs[ri].append(eng.bitstream(sid))
             done = left <= 0
             if done.any():
                 for sid in sids[done].tolist():
@@ -188,7 +199,7 @@ class ServingLoop:
                              request_latency_ms_p99=float(np.percentile(rl, 99)),
                              input_wait_ms_p50=float(np.percentile(iw, 50)),
                              input_wait_ms_p99=float(np.percentile(iw, 99)), max_concurrent=max_conc,
-                             steps=steps)
+                             steps=steps, delivered_bytes=delivered)
         return (stats, outputs) if collect else stats
 
     def close(self):

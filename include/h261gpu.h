@@ -204,6 +204,19 @@ H261G_API int h261g_stream_read_me(h261g_engine_t *e, int32_t id, me_gpu_mb_t *m
  * buffer, it is written here on the host from the device's levels and side
  * information (the same bytes) */
 H261G_API int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int32_t cap);
+/* bulk bitstream delivery: queues, after the work queued so far, a copy of
+ * the last frame's bitstream of each listed stream into pinned host memory
+ * owned by the engine, and returns at once.  A later fetch first waits for
+ * this one. */
+H261G_API int h261g_engine_fetch_bitstreams(h261g_engine_t *e, const int32_t *ids, int32_t n);
+/* waits for the last fetch; returns its stream count n.  Stream ids[i]'s
+ * bytes are data[offsets[i] .. offsets[i] + sizes[i]); sizes[i] < 0: that
+ * frame is not in the host buffer -- it outgrew its device buffer, device
+ * bitstreams are off, or the host buffer was full (it grows for the next
+ * fetch) -- read it with h261g_stream_read_bitstream.  Valid until the next
+ * fetch. */
+H261G_API int h261g_engine_fetched(h261g_engine_t *e, const uint8_t **data, const int32_t **sizes,
+                                   const int32_t **offsets);
 /* the current input planes, for checking device-generated content
  * (synchronises the queued input generation and frame steps) */
 H261G_API int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);

@@ -126,3 +126,34 @@ def test_device_bitstream_option_needs_no_open_stream():
             eng.config(nat.OPT_DEVICE_BITSTREAM, 0)
     finally:
         eng.destroy()
+
+
+def test_bulk_fetch_matches_per_stream_reads():
+    """h261g_engine_fetch_bitstreams delivers every listed stream's last
+    frame in one queued operation, the same bytes as the per-stream read,
+    including a frame that outgrew its device buffer (host-written)"""
+    eng = Engine(max_streams=16)
+    try:
+        sids = []
+        for i in range(12):
+            size = CIF if i % 3 else QCIF
+            sid = eng.open(*size, bitrate=int(96000 + 40000 * i), vbv_size=int(96000 + 40000 * i))
+            eng.generate_input(sid, 900 + i, 6, sigma=4, motion=15, local=bool(i % 2))
+            sids.append(sid)
+        big = eng.open(*QCIF, fixed_quant=1)
+        big_frame = content_frame(77, *QCIF, 0, sigma=16, motion=3)
+        for t in range(3):
+            announce(f"bulk fetch step {t}")
+            eng.put_frame(big, *big_frame)
+            eng.step()
+            order = sids[::-1] + [big] + sids[:3]
+            eng.fetch_bitstreams(order)
+            got = eng.fetched()
+            eng.sync()
+            want = [eng.bitstream(sid) for sid in order]
+            assert got == want, t
+            assert len(got[len(sids)]) > 32 * 1024 or t > 0
+        eng.fetch_bitstreams([])
+        assert eng.fetched() == []
+    finally:
+        eng.destroy()
