@@ -121,10 +121,11 @@ def open_streams(eng, specs, sigma=2, frames=None):
     return ids
 
 
-def run_job(eng, ids, specs, steps, start=0, compact_every=60, after_chunk=None):
+def run_job(eng, ids, specs, steps, start=0, compact_every=30, after_chunk=None):
     """frame steps start..start+steps-1 (graph replay; eager with events when
     the engine profiles), closing finished streams and re-packing the MB
-    space every compact_every steps.  Returns (seconds, frames, mbs)."""
+    space every compact_every steps (closing and re-packing are queued
+    behind the steps: nothing waits).  Returns (seconds, frames, mbs)."""
     lengths = np.array([s["length"] for s in specs])
     nmb = np.array([(s["size"][0] // 16) * (s["size"][1] // 16) for s in specs])
     done = start
@@ -140,7 +141,6 @@ def run_job(eng, ids, specs, steps, start=0, compact_every=60, after_chunk=None)
             after_chunk(done)
         fin = open_mask & (lengths <= done)
         if fin.any() and done < end:
-            eng.sync()
             for i in np.nonzero(fin)[0]:
                 eng.close(ids[i])
             open_mask &= ~fin
