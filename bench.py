@@ -46,22 +46,26 @@ ME_BYTES = {396: 352 * 288 * 2 + 44 * 396, 99: 176 * 144 * 2 + 44 * 99}
 MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 384 + 8)}
 
 
-# HBM bytes per macroblock actually moved by each kernel: FETCH_SIZE and
-# WRITE_SIZE per dispatch from two rocprofv3 --pmc passes over
-# `bench.py --steps 8 --no-cpu --no-plain --no-extra --copy-probe` (all 512
-# streams active, 126,720 MBs per dispatch), each scaled by what the same
-# counter reads for the 256 MiB device copy of --copy-probe, divided by the
-# dispatch's MB count (profiles/batched_512_streams_hbm_traffic.md: motion
-# estimation 508.6 MB read + 16.3 MB written, the coder 99.0 + 133.0 MB, the
-# bitstream writer 12.7 + 6.7 MB).
-TRAFFIC_PER_MB = {"me_kernel": (508.65e6 + 16.29e6) / 126720, "mb_kernel": (99.05e6 + 132.96e6) / 126720,
-                  "bitstream_kernel": (12.72e6 + 6.68e6) / 126720}
+# HBM bytes per macroblock actually moved by each kernel, with and without
+# its XCD remap: FETCH_SIZE and WRITE_SIZE per dispatch from two rocprofv3
+# --pmc passes over `bench.py --steps 8 --no-cpu --no-plain --no-extra
+# --copy-probe --shape 0,R` (R = 3 and 2; all 512 streams active, 126,720 MBs
+# per dispatch), each scaled by what the same counter reads for the 256 MiB
+# device copy of --copy-probe, divided by the dispatch's MB count
+# (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation:
+# 98.3 MB read + 9.7 MB written remapped, 509.3 + 16.3 MB not; the coder
+# (remapped) 105.2 + 135.1 MB; the bitstream writer (not remapped)
+# 21.6 + 6.7 MB.
+TRAFFIC_PER_MB = {("me_kernel", True): (98.31e6 + 9.70e6) / 126720, ("me_kernel", False): (509.25e6 + 16.29e6) / 126720,
+                  ("mb_kernel", True): (105.20e6 + 135.09e6) / 126720,
+                  ("bitstream_kernel", False): (21.55e6 + 6.68e6) / 126720}
+REMAP_BIT = {"me_kernel": 1, "mb_kernel": 2, "bitstream_kernel": 4}
 
 
-def traffic_per_launch(kernel, mbs_per_launch):
-    """HBM traffic of an average launch: measured bytes per MB times the
-    launch's MBs"""
-    t = TRAFFIC_PER_MB.get(kernel)
+def traffic_per_launch(kernel, mbs_per_launch, xcd_remap):
+    """HBM traffic of an average launch: measured bytes per MB (for the
+    kernel's XCD remap setting) times the launch's MBs"""
+    t = TRAFFIC_PER_MB.get((kernel, bool(xcd_remap & REMAP_BIT[kernel])))
     return None if t is None else int(t * mbs_per_launch)
 
 
@@ -280,6 +284,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the hard-motion and serving lines")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--shape", default=None,
+                    help="MBS_PER_WAVE,XCD_REMAP: a fixed launch shape instead of the autotuner's (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -301,7 +307,11 @@ def main():
     # launch shape (coder MBs per wave, XCD remap) tuned on a sample of this
     # workload before anything is timed
     from h261enc_cuda_amd import autotune
-    tuned = autotune.tune([s["size"] for s in specs])
+    if args.shape:
+        mpw, remap = (int(v) for v in args.shape.split(","))
+        tuned = {"mbs_per_wave": mpw, "xcd_remap": remap, "fixed": True}
+    else:
+        tuned = autotune.tune([s["size"] for s in specs])
 
     eng = Engine(max_streams=len(specs))
     autotune.apply(eng, tuned)
@@ -367,18 +377,18 @@ def main():
     achieved = job_k[key + "_gbs"]
     roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_per_launch(dominant, job_k["mbs"]),
+            "traffic": traffic_per_launch(dominant, job_k["mbs"], tuned["xcd_remap"]),
             "avg_launch_ms": round(job_k[key + "_ms"], 4),
             "algorithmic_bytes_per_launch": int(job_k[key + "_bytes"]), "launches": job_k["launches"]}
     for name, f in (("job_average", job_k), ("full_load", full_k)):
         roof[name] = {"launches": f["launches"],
                       "mbs_per_launch": round(f["mbs"], 1),
                       "me_kernel": {"avg_launch_ms": round(f["me_ms"], 4), "hbm_gbs": round(f["me_gbs"], 1),
-                                    "traffic": traffic_per_launch("me_kernel", f["mbs"]),
+                                    "traffic": traffic_per_launch("me_kernel", f["mbs"], tuned["xcd_remap"]),
                                     "sad_rate_tops": round(f["sad_rate"] / 1e12, 3),
                                     "sad_frac": round(f["sad_rate"] / SAD_PEAK, 4)},
                       "mb_kernel": {"avg_launch_ms": round(f["mb_ms"], 4), "hbm_gbs": round(f["mb_gbs"], 1),
-                                    "traffic": traffic_per_launch("mb_kernel", f["mbs"]),
+                                    "traffic": traffic_per_launch("mb_kernel", f["mbs"], tuned["xcd_remap"]),
                                     "hbm_frac": round(f["mb_gbs"] / HBM_PEAK_GBS, 4)},
                       "frame_tail_ms_per_step": round(f["other_ms"], 4),
                       "bitstream_kernel_ms_per_step": round(f["bits_ms"], 4)}

@@ -15,8 +15,12 @@ mix of stream sizes, all streams active, inter frames -- with HIP events
 around every launch of the step, and keeps the fastest: first over
 `mbs_per_wave` at the default remap, then over the remap at the best
 `mbs_per_wave` (coordinate descent; the two knobs act on mostly different
-kernels).  One engine serves every trial; each trial first runs one
-untimed step with its settings.
+kernels).  Among remaps within `tie` (1%) of the fastest, one that remaps
+motion estimation is preferred: its time is the same within noise, and it
+reads about 7x fewer HBM bytes (neighbouring MBs' search windows meet in one
+XCD's L2), which leaves the memory system to concurrent work.  One engine
+serves every trial; each trial first runs one untimed step with its
+settings.
 """
 from __future__ import annotations
 
@@ -34,7 +38,7 @@ def _step_ms(t: dict) -> float:
 
 
 def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 2, 4),
-         xcd_remap: Iterable[int] = tuple(range(8)), steps: int = 4, seed: int = 99) -> dict:
+         xcd_remap: Iterable[int] = tuple(range(8)), steps: int = 4, seed: int = 99, tie: float = 0.01) -> dict:
     """sizes: (width, height) of every stream of the sample.  Returns
     {"mbs_per_wave": best, "xcd_remap": best, "step_ms": its time per step,
     "trials": [{"mbs_per_wave", "xcd_remap", "step_ms", kernel times}]}."""
@@ -70,9 +74,12 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
                            "bits_ms": round(t["bits_ms"] / n, 4)})
             return ms
 
-        default_remap = 2
+        default_remap = 3
         best_mpw = min(mpw_c, key=lambda m: trial(m, default_remap))
-        best_remap = min(remap_c, key=lambda r: trial(best_mpw, r))
+        times = {r: trial(best_mpw, r) for r in remap_c}
+        fastest = min(times.values())
+        close = [r for r in remap_c if times[r] <= fastest * (1 + tie)]
+        best_remap = next((r for r in close if r & 1), min(close, key=lambda r: times[r]))
     finally:
         eng.destroy()
     best = min((t for t in trials if t["mbs_per_wave"] == best_mpw and t["xcd_remap"] == best_remap),

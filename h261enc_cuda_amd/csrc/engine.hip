@@ -158,12 +158,12 @@ struct h261g_engine {
     int me_variant = kMeTwoStage;
     int fused_tails = 1;
     /* bit 0: motion estimation, bit 1: macroblock coder, bit 2: bitstream
-     * writer (measured: the writer is faster without).  Measured at 512
-     * streams: the remap cuts the coder's L2 misses ~9x and its time ~4% (at
-     * 4 MBs per wave); it cuts motion estimation's misses ~7.6x but not its
-     * time (that kernel is VALU/LDS bound, its re-reads are absorbed below
-     * L2), so only the coder is remapped by default */
-    int xcd_remap = 2;
+     * writer.  Measured at 512 streams: the remap cuts the coder's HBM reads
+     * ~9x and its time ~4-14% (at 4 MBs per wave); it cuts motion
+     * estimation's HBM reads ~7.6x (509 -> 67 MB per launch) at the same
+     * time within noise (that kernel is VALU/LDS bound); the writer is ~20%
+     * faster without it.  Default: motion estimation and coder. */
+    int xcd_remap = 3;
     /* 1: bitstreams written on the device into a per-stream buffer of
      * kBitsBytes*; 2: buffers of the worst-case frame size; 0: written on
      * the host from the levels when read */

@@ -132,15 +132,23 @@ static_assert(sizeof(RcState) == 72, "RcState layout");
 
 /* Workgroups are dispatched to the 8 XCDs of the device in turn, each XCD
  * with its own L2.  Neighbouring macroblocks share reference and search
- * window lines, so a launch of n logical workgroups is padded to a multiple
- * of 8 and workgroup i is given logical index (i % 8) * per + i / 8: every
- * XCD codes a contiguous run of macroblocks.  Indexes >= n exit. */
+ * window lines, so logical workgroups are dealt to the XCDs in chunks of
+ * kXcdChunk consecutive ones: workgroup i (XCD i % 8, its k = i / 8-th
+ * there) is given logical index ((k / C) * 8 + i % 8) * C + k % C.  The grid
+ * is padded to a multiple of 8 C; indexes >= n exit.  Chunks rather than one
+ * contiguous run per XCD keep the XCDs' loads equal when a range of
+ * macroblocks is idle (a finished stream awaiting re-packing). */
 constexpr int kXcds = 8;
-H261G_HD inline int xcd_grid(int n) { return ((n + kXcds - 1) / kXcds) * kXcds; }
+#ifndef H261G_XCD_CHUNK
+#define H261G_XCD_CHUNK 64
+#endif
+constexpr int kXcdChunk = H261G_XCD_CHUNK;
+H261G_HD inline int xcd_grid(int n) { return ((n + kXcds * kXcdChunk - 1) / (kXcds * kXcdChunk)) * kXcds * kXcdChunk; }
 H261G_HD inline int xcd_logical(int i, int n)
 {
-    const int per = (n + kXcds - 1) / kXcds;
-    return (i % kXcds) * per + i / kXcds;
+    (void)n;
+    const int k = i / kXcds;
+    return ((k / kXcdChunk) * kXcds + i % kXcds) * kXcdChunk + k % kXcdChunk;
 }
 H261G_HD inline int launch_grid(int n, bool remap) { return remap ? xcd_grid(n) : n; }
 H261G_HD inline int launch_index(int i, int n, bool remap) { return remap ? xcd_logical(i, n) : i; }

@@ -127,7 +127,7 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                     bitstream writer; a set bit
                                     gives each XCD a contiguous run of macroblocks (neighbours
                                     share its L2), a clear bit has workgroup i code macroblock i.
-                                    Default 2. */
+                                    Default 3. */
 #define H261G_OPT_ASYNC_INPUT 5  /* 1: h261g_stream_generate_input on a new stream generates on a
                                     second HIP stream, overlapped with frame steps; the stream
                                     joins the first step after its input is ready.  0 (default):

@@ -52,17 +52,36 @@ def test_product_does_not_reference_the_oracle():
                 assert "oracle" not in text.lower(), f"{f} mentions the oracle"
 
 
-def test_xcd_remap_is_a_bijection():
-    """The XCD-aware workgroup order (h261gpu_internal.h xcd_logical) covers
-    every logical index exactly once for any launch size."""
-    def grid(n):
-        return ((n + 7) // 8) * 8
-
-    def logical(i, n):
-        per = (n + 7) // 8
-        return (i % 8) * per + i // 8
-
-    for n in list(range(1, 200)) + [396, 99, 126720, 63360 + 5]:
-        seen = [logical(i, n) for i in range(grid(n))]
-        inside = sorted(g for g in seen if g < n)
-        assert inside == list(range(n)), n
+def test_xcd_remap_is_a_bijection(tmp_path):
+    """The XCD-aware workgroup order (launch_grid / launch_index of
+    h261gpu_internal.h, compiled here by the host compiler) covers every
+    logical index exactly once for any launch size, padding aside; the
+    chunks of consecutive indexes alternate between the 8 XCDs."""
+    import subprocess
+    src = tmp_path / "xcd.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include "h261gpu_internal.h"\n'
+        "int main() {\n"
+        "  const int sizes[] = {1, 7, 8, 9, 63, 64, 65, 99, 396, 511, 512, 513, 4095, 31680, 63365, 126720};\n"
+        "  for (int n : sizes) {\n"
+        "    const int g = h261g::launch_grid(n, true);\n"
+        "    std::printf(\"%d %d\", n, g);\n"
+        "    for (int i = 0; i < g; i++) std::printf(\" %d\", h261g::launch_index(i, n, true));\n"
+        "    std::printf(\"\\n\");\n"
+        "  }\n"
+        "  std::printf(\"chunk %d\\n\", h261g::kXcdChunk);\n"
+        "}\n")
+    exe = tmp_path / "xcd"
+    csrc = os.path.join(ROOT, "h261enc_cuda_amd", "csrc")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", csrc, str(src), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    chunk = int(lines[-2].split()[1])
+    for line in lines[:-2]:
+        v = [int(x) for x in line.split()]
+        n, g, idx = v[0], v[1], v[2:]
+        assert len(idx) == g and g % 8 == 0 and g - n < 8 * chunk, n
+        assert sorted(idx) == list(range(g)), n  # a permutation of the padded grid
+        # workgroup i runs on XCD i % 8; its logical index lies in a chunk
+        # dealt to that XCD
+        for i, l in enumerate(idx):
+            assert (l // chunk) % 8 == i % 8, (n, i)
