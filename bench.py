@@ -327,9 +327,14 @@ def main():
     value = cif_equiv / dt
 
     # pass B: the same W + K steps again, each kernel bracketed by HIP events
-    # on the engine stream; the averages are over the same launches as pass A
+    # on the engine stream; the averages are over the same launches as pass A.
+    # Each kernel runs alone here: the bitstream writer at the end of its own
+    # step rather than beside the next step's motion estimation, so no event
+    # span covers two kernels
+    from h261enc_cuda_amd import _native as nat
     eng = Engine(max_streams=len(specs))
     autotune.apply(eng, tuned)
+    eng.config(nat.OPT_OVERLAP_WRITER, 0)
     ids = open_streams(eng, specs)
     for _ in range(W):
         eng.step()
@@ -348,7 +353,6 @@ def main():
     # pass C: the same job without device bitstreams -- levels, side
     # information, exact frame sizes and reconstructions ready (the device
     # layer's boundary; the bitstream is then written on the host when read)
-    from h261enc_cuda_amd import _native as nat
     eng = Engine(max_streams=len(specs))
     eng.config(nat.OPT_DEVICE_BITSTREAM, 0)
     autotune.apply(eng, tuned)

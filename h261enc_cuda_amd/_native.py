@@ -59,6 +59,7 @@ OPT_FUSED_TAILS = 3
 OPT_XCD_REMAP = 4
 OPT_ASYNC_INPUT = 5
 OPT_DEVICE_BITSTREAM = 6
+OPT_OVERLAP_WRITER = 7
 
 
 class StreamParams(ctypes.Structure):

@@ -153,6 +153,14 @@ struct h261g_engine {
      * the event marking the compute work a generation must follow */
     hipStream_t in_stream = nullptr;
     hipEvent_t compute_ev = nullptr;
+    /* the bitstream writer of a step runs on its own stream beside the next
+     * step's motion estimation (which does not touch its inputs); the next
+     * step's coder, which overwrites them, waits for it.  writer_pending:
+     * the last step's writer has not been launched yet */
+    hipStream_t wr_stream = nullptr;
+    hipEvent_t wr_fork = nullptr, wr_join = nullptr;
+    int overlap_writer = 1;
+    bool writer_pending = false;
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_variant = kMeTwoStage;
@@ -395,6 +403,30 @@ int read_oldest_times(h261g_engine *e);
 constexpr int kFillWaves = 256 * 4 * 5;
 
 /* one frame step: three launches (default), or the six standalone launches */
+/* the bitstream writer over the whole MB space (MBs not placed by a frame
+ * tail since the last writer are skipped) */
+BitLaunch writer_launch(const h261g_engine *e)
+{
+    BitLaunch wl{};
+    wl.mb_stream = e->mb_stream;
+    wl.n_mb = e->mb_used;
+    wl.seq = e->seq;
+    wl.hdr = e->hdr;
+    wl.side = e->side;
+    wl.levels = e->levels;
+    wl.xcd_remap = (e->xcd_remap >> 2) & 1;
+    return wl;
+}
+
+/* the last step's bitstreams, now, on the engine stream: before anything
+ * reads them or changes the MB space they are indexed by */
+int flush_writer(h261g_engine *e)
+{
+    if (!e->writer_pending) return H261G_OK;
+    e->writer_pending = false;
+    return launch_bitstream(writer_launch(e), e->stream) ? H261G_ERR_DEVICE : H261G_OK;
+}
+
 int enqueue_step(h261g_engine *e, bool timed)
 {
     if (timed && e->ev_recorded - e->ev_read == h261g_engine::kEvRing) {
@@ -417,7 +449,17 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.variant = e->me_variant;
     ml.fuse_tail = fused;
     ml.xcd_remap = e->xcd_remap & 1;
+    /* the previous step's bitstreams beside this step's motion estimation
+     * (with nothing pending every MB is skipped) */
+    const bool overlap = e->device_bits && e->overlap_writer;
+    if (overlap) {
+        if (!ok(hipEventRecord(e->wr_fork, st), "event") || !ok(hipStreamWaitEvent(e->wr_stream, e->wr_fork, 0), "wait") ||
+            launch_bitstream(writer_launch(e), e->wr_stream, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr) ||
+            !ok(hipEventRecord(e->wr_join, e->wr_stream), "event"))
+            return H261G_ERR_DEVICE;
+    }
     if (launch_me(ml, st, timed ? ev[0] : nullptr, timed ? ev[1] : nullptr)) return H261G_ERR_DEVICE;
+    if (overlap && !ok(hipStreamWaitEvent(st, e->wr_join, 0), "wait")) return H261G_ERR_DEVICE;
     if (!fused && launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
     MbLaunch bl{};
     bl.streams = e->desc;
@@ -450,20 +492,14 @@ int enqueue_step(h261g_engine *e, bool timed)
         if (launch_advance(e->desc, e->seq, ns, st)) return H261G_ERR_DEVICE;
         if (timed) (void)hipEventRecord(ev[5], st);
     }
-    if (hdr) {
-        BitLaunch wl{};
-        wl.mb_stream = e->mb_stream;
-        wl.n_mb = n_mb;
-        wl.seq = e->seq;
-        wl.hdr = hdr;
-        wl.side = e->side;
-        wl.levels = e->levels;
-        wl.xcd_remap = (e->xcd_remap >> 2) & 1;
-        if (launch_bitstream(wl, st, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr)) return H261G_ERR_DEVICE;
-    } else if (timed) {
+    if (hdr && !overlap) {
+        if (launch_bitstream(writer_launch(e), st, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr))
+            return H261G_ERR_DEVICE;
+    } else if (!hdr && timed) {
         (void)hipEventRecord(ev[6], st);
         (void)hipEventRecord(ev[7], st);
     }
+    e->writer_pending = overlap;
     if (timed) e->ev_recorded++;
     return H261G_OK;
 }
@@ -509,6 +545,7 @@ extern "C" {
 int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
 {
     if (!e) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     switch (option) {
     case H261G_OPT_ME_VARIANT:
         if (value != kMeTwoStage && value != kMeSurface) return H261G_ERR_ARG;
@@ -530,6 +567,10 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         if (value != 0 && value != 1) return H261G_ERR_ARG;
         e->async_input = value;
         return H261G_OK;
+    case H261G_OPT_OVERLAP_WRITER:
+        if (value != 0 && value != 1) return H261G_ERR_ARG;
+        e->overlap_writer = value;
+        break;
     case H261G_OPT_DEVICE_BITSTREAM:
         if (value < 0 || value > 2) return H261G_ERR_ARG;
         /* bitstream buffers are sized at open */
@@ -580,6 +621,9 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
     e->slots.resize(max_streams);
     bool good = ok(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "stream") &&
                 ok(hipStreamCreateWithFlags(&e->in_stream, hipStreamNonBlocking), "stream") &&
+                ok(hipStreamCreateWithFlags(&e->wr_stream, hipStreamNonBlocking), "stream") &&
+                ok(hipEventCreateWithFlags(&e->wr_fork, hipEventDisableTiming), "event") &&
+                ok(hipEventCreateWithFlags(&e->wr_join, hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->compute_ev, hipEventDisableTiming), "event") &&
                 ok(hipMalloc((void **)&e->desc, sizeof(StreamDesc) * max_streams), "hipMalloc") &&
                 ok(hipMalloc((void **)&e->seq, sizeof(SeqState) * max_streams), "hipMalloc") &&
@@ -602,6 +646,7 @@ int h261g_engine_destroy(h261g_engine_t *e)
     if (!e) return H261G_ERR_ARG;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->in_stream) (void)hipStreamSynchronize(e->in_stream);
+    if (e->wr_stream) (void)hipStreamSynchronize(e->wr_stream);
     for (auto &s : e->slots)
         if (s.gen_ev) (void)hipEventDestroy(s.gen_ev);
     for (auto &c : e->chunks) hipFree(c.base);
@@ -632,6 +677,9 @@ int h261g_engine_destroy(h261g_engine_t *e)
             if (e->ev[r][i]) (void)hipEventDestroy(e->ev[r][i]);
     if (e->compute_ev) (void)hipEventDestroy(e->compute_ev);
     if (e->in_stream) (void)hipStreamDestroy(e->in_stream);
+    if (e->wr_stream) (void)hipStreamDestroy(e->wr_stream);
+    if (e->wr_fork) (void)hipEventDestroy(e->wr_fork);
+    if (e->wr_join) (void)hipEventDestroy(e->wr_join);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return H261G_OK;
@@ -649,6 +697,7 @@ int h261g_stream_open(h261g_engine_t *e, const h261g_stream_params_t *p)
             break;
         }
     if (id < 0) return H261G_ERR_FULL;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     const int64_t fp = stream_footprint(p->width, p->height, e->device_bits);
     if (e->budget && e->used + fp > e->budget) return H261G_ERR_FULL;
     Slot &s = e->slots[id];
@@ -753,6 +802,7 @@ int h261g_stream_close(h261g_engine_t *e, int32_t id)
 {
     Slot *s = get_slot(e, id);
     if (!s) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     /* an input still being generated finishes before its buffer is released */
     if (s->gen_pending && !ok(hipStreamWaitEvent(e->stream, s->gen_ev, 0), "wait")) return H261G_ERR_DEVICE;
     hipEvent_t gen_ev = s->gen_ev;
@@ -777,6 +827,7 @@ int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_t *y, cons
 {
     Slot *s = get_slot(e, id);
     if (!s || !y || !cb || !cr || s->seq) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     const size_t ysz = (size_t)s->width * s->height, csz = ysz / 4;
     int32_t flags = kStreamActive | kStreamDeviceQuant;
     bool good = ok(hipMemcpyAsync(s->input, y, ysz, hipMemcpyHostToDevice, e->stream), "hipMemcpy") &&
@@ -800,6 +851,7 @@ int h261g_stream_frames_done(h261g_engine_t *e, int32_t id)
 {
     Slot *s = get_slot(e, id);
     if (!s) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     StreamDesc d;
     if (!ok(hipStreamSynchronize(e->stream), "sync") ||
         !ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy"))
@@ -815,6 +867,7 @@ int h261g_stream_frames_done(h261g_engine_t *e, int32_t id)
 int h261g_engine_compact(h261g_engine_t *e)
 {
     if (!e) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     int n = 0;
     for (int i = 0; i < e->max_streams; i++)
         if (e->slots[i].open) n += e->slots[i].n_mb;
@@ -868,6 +921,7 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
 {
     Slot *s = get_slot(e, id);
     if (!s || !c || n_frames <= 0 || s->seq) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     const int64_t fsz = (int64_t)s->width * s->height * 3 / 2;
     const int64_t bytes = fsz * n_frames;
     if (e->budget && e->used + bytes > e->budget) return H261G_ERR_FULL;
@@ -1003,6 +1057,7 @@ int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
 int h261g_engine_sync(h261g_engine_t *e)
 {
     if (!e) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     return ok(hipStreamSynchronize(e->stream), "sync") ? H261G_OK : H261G_ERR_DEVICE;
 }
 
@@ -1098,6 +1153,7 @@ int h261g_stream_read_bitstream(h261g_engine_t *e, int32_t id, uint8_t *out, int
 {
     Slot *s = get_slot(e, id);
     if (!s || !out || cap < 0) return H261G_ERR_ARG;
+    if (flush_writer(e) || (s->bits && !ok(hipStreamSynchronize(e->stream), "sync"))) return H261G_ERR_DEVICE;
     if (s->bits) {
         /* written by the device when the frame fit the stream's buffer */
         RcState rc;
@@ -1130,6 +1186,7 @@ int h261g_engine_fetch_bitstreams(h261g_engine_t *e, const int32_t *ids, int32_t
     if (!e || n < 0 || (n && !ids)) return H261G_ERR_ARG;
     for (int i = 0; i < n; i++)
         if (!get_slot(e, ids[i])) return H261G_ERR_ARG;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
     /* the previous fetch's buffers are rewritten below */
     if (e->fetch_pending && !ok(hipEventSynchronize(e->fetch_ev), "event")) return H261G_ERR_DEVICE;
     e->fetch_pending = false;
