@@ -48,6 +48,9 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
         return _CACHE[key]
     frames = 2 + (steps + 1) * (len(mpw_c) + len(remap_c))
     eng = Engine(max_streams=len(sizes))
+    # kernels alone (the writer at the end of its step, not beside the next
+    # step's motion estimation), so the event spans add up to the step
+    eng.config(nat.OPT_OVERLAP_WRITER, 0)
     trials: List[dict] = []
     try:
         for i, (w, h) in enumerate(sizes):

@@ -159,7 +159,7 @@ struct h261g_engine {
      * the last step's writer has not been launched yet */
     hipStream_t wr_stream = nullptr;
     hipEvent_t wr_fork = nullptr, wr_join = nullptr;
-    int overlap_writer = 1;
+    int overlap_writer = 2; /* 0 off, 1 on, 2 when a launch fills the device */
     bool writer_pending = false;
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
@@ -451,7 +451,7 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.xcd_remap = e->xcd_remap & 1;
     /* the previous step's bitstreams beside this step's motion estimation
      * (with nothing pending every MB is skipped) */
-    const bool overlap = e->device_bits && e->overlap_writer;
+    const bool overlap = e->device_bits && (e->overlap_writer == 1 || (e->overlap_writer == 2 && n_mb >= kFillWaves));
     if (overlap) {
         if (!ok(hipEventRecord(e->wr_fork, st), "event") || !ok(hipStreamWaitEvent(e->wr_stream, e->wr_fork, 0), "wait") ||
             launch_bitstream(writer_launch(e), e->wr_stream, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr) ||
@@ -568,7 +568,7 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         e->async_input = value;
         return H261G_OK;
     case H261G_OPT_OVERLAP_WRITER:
-        if (value != 0 && value != 1) return H261G_ERR_ARG;
+        if (value < 0 || value > 2) return H261G_ERR_ARG;
         e->overlap_writer = value;
         break;
     case H261G_OPT_DEVICE_BITSTREAM:

@@ -138,10 +138,11 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                         worst-case frame size; 0: no device bitstream (written on the
                                         host when read).  Set while no stream is open
                                         (H261G_ERR_STATE otherwise). */
-#define H261G_OPT_OVERLAP_WRITER 7 /* 1 (default): a step's bitstream writer runs beside the next step's
-                                      motion estimation, on a second HIP stream (h261g_engine_sync, reads
-                                      and stream changes launch a pending writer first); 0: at the end of
-                                      its own step */
+#define H261G_OPT_OVERLAP_WRITER 7 /* 1: a step's bitstream writer runs beside the next step's motion
+                                      estimation, on a second HIP stream (h261g_engine_sync, reads and
+                                      stream changes launch a pending writer first); 0: at the end of its
+                                      own step; 2 (default): 1 when a step has enough macroblocks to
+                                      fill the device, else 0 (a short step is latency bound) */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */

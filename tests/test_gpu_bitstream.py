@@ -157,3 +157,40 @@ def test_bulk_fetch_matches_per_stream_reads():
         assert eng.fetched() == []
     finally:
         eng.destroy()
+
+
+def test_overlapped_writer_equals_in_step_writer():
+    """The writer beside the next step's motion estimation (default) and at
+    the end of its own step give the same bytes, across graph replays,
+    closes, re-packing and opens between steps"""
+    def job(overlap):
+        eng = Engine(max_streams=12)
+        out = []
+        try:
+            eng.config(nat.OPT_OVERLAP_WRITER, overlap)
+            lengths = [3, 9, 5, 9, 2, 7, 9, 4]
+            sids = []
+            for i, n in enumerate(lengths):
+                sid = eng.open(*(CIF if i % 2 else QCIF), bitrate=160000, vbv_size=160000)
+                eng.generate_input(sid, 300 + i, n, sigma=4, motion=15, local=bool(i % 3 == 0), scene_cut=4)
+                sids.append(sid)
+            live = dict(zip(sids, lengths))
+            done = 0
+            for chunk in (1, 2, 3, 3):
+                announce(f"overlapped writer {overlap} chunk {chunk} at {done}")
+                eng.run(chunk)
+                done += chunk
+                eng.sync()
+                out.append([(sid, eng.info(sid).frame_num, eng.bitstream(sid)) for sid in sorted(live)])
+                for sid in [s for s, n in live.items() if n <= done]:
+                    eng.close(sid)
+                    del live[sid]
+                eng.compact()
+                if chunk == 2:  # a new stream joins after re-packing
+                    sid = eng.open(*QCIF, bitrate=96000, vbv_size=96000)
+                    eng.generate_input(sid, 777, 6, sigma=4, motion=15)
+                    live[sid] = done + 6
+            return out
+        finally:
+            eng.destroy()
+    assert job(1) == job(0)
