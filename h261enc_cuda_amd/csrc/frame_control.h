@@ -7,12 +7,13 @@
  *                       fixed-quant override, h261_init_picture's clamp
  *                       (src/h261_ratectrl.c:130-195, 225-245,
  *                       src/h261_picture.c:25-33)
- *   frame_bits_wave     exact size of the coded frame: picture and GOB
+ *   frame_bits_group    exact size of the coded frame: picture and GOB
  *                       headers plus, per coded MB, the MBA / MTYPE / MQUANT
  *                       / MVD bits the writer's GOB state produces
  *                       (src/h261_gob.c:58-199, src/h261_macroblock.c:94-245)
- *                       added to the MB coder's CBP + TCOEFF count.  One
- *                       wave: lane i = MB i of the GOB; the previous coded MB
+ *                       added to the MB coder's CBP + TCOEFF count.  The
+ *                       waves of a workgroup share out the GOBs; lane i = MB
+ *                       i of a GOB; the previous coded MB
  *                       (MBA increment), the last MB with coefficients
  *                       (MQUANT state) and the MV predictor (MB i-1, reset at
  *                       MBs 1, 12, 23) come from ballots and lane shuffles.
@@ -20,7 +21,7 @@
  *   denoise_update_lane macroblocks_update_dct_denoise (mbgpu:478-503)
  *   advance_one         the reconstruction becomes the next reference, the
  *                       next input frame is selected
- *   frame_tail_wave     the four above that end a frame, in one wave
+ *   frame_tail_group    the four above that end a frame, in one workgroup
  *
  * Fusing stays fence-free: an agent-scope fence on this device writes back
  * and invalidates the XCD's L2, which costs far more than a launch when
@@ -167,20 +168,25 @@ __device__ __forceinline__ void or_bits(uint32_t *words, int pos, uint32_t code,
     }
 }
 
-/* all 64 lanes call; returns the frame's total bits on every lane.  The side
- * information of every GOB is loaded before any is scanned, so the loads
- * overlap instead of costing one memory latency per GOB, and the code tables
- * are copied to LDS first, so no lookup inside the scan waits on memory.
+/* the waves of a frame-tail workgroup share out the GOBs (wave w takes GOBs
+ * w, w + kTailWaves, ...) and each scans its own: lane i = MB i of the GOB.
+ * Every thread of the workgroup calls; returns the frame's total bits on
+ * every thread.  The side information of a wave's GOBs is loaded before any
+ * is scanned, and the code tables are copied to LDS first, so no step of a
+ * scan waits on memory.
  *
  * With hdr and bits (the stream's bitstream words, at least the frame's
- * size): each coded MB's bit position and header codes (MBA, MTYPE, MQUANT,
- * MVD, CBP) go to hdr for the bitstream writer, the frame's words are zeroed
- * and the picture and GOB headers written (src/h261_picture.c:38-71,
- * src/h261_gob.c:58-199, src/h261_macroblock.c:94-245). */
-__device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide *side, int picq, int lane,
-                                               MbHdr *hdr = nullptr, uint32_t *bits = nullptr, int bits_words = 0)
+ * size): each coded MB's bit position, header codes (MBA, MTYPE, MQUANT,
+ * MVD, CBP) and total bits go to hdr for the bitstream writer, the frame's
+ * words are zeroed and the picture and GOB headers written
+ * (src/h261_picture.c:38-71, src/h261_gob.c:58-199,
+ * src/h261_macroblock.c:94-245). */
+constexpr int kTailWaves = 4;
+__device__ __forceinline__ int frame_bits_group(const StreamDesc &d, const MbSide *side, int picq, int wave, int lane,
+                                                MbHdr *hdr = nullptr, uint32_t *bits = nullptr, int bits_words = 0)
 {
     constexpr int kMaxGob = 12;
+    constexpr int kPer = (kMaxGob + kTailWaves - 1) / kTailWaves; /* GOBs per wave */
     const bool cif = d.mb_w == 22;
     const int n_gob = cif ? 12 : 3;
     const int gob_cols = cif ? 2 : 1;
@@ -190,22 +196,18 @@ __device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide
      * GOBs past the last read a valid entry and discard it), so all loads
      * issue back to back */
     const int ml = valid ? lane : 32;
-    uint64_t raw[kMaxGob];
+    uint64_t raw[kPer];
 #pragma unroll
-    for (int g = 0; g < kMaxGob; g++) {
+    for (int k = 0; k < kPer; k++) {
+        const int g = wave + k * kTailWaves;
         const int gg = g < n_gob ? g : n_gob - 1;
         const int gx = (gg % gob_cols) * 11, gy = (gg / gob_cols) * 3;
-        raw[g] = *(const uint64_t *)(side + d.mb_base + (gy + ml / 11) * d.mb_w + gx + ml % 11);
+        raw[k] = *(const uint64_t *)(side + d.mb_base + (gy + ml / 11) * d.mb_w + gx + ml % 11);
     }
-    MbSide sds[kMaxGob];
-#pragma unroll
-    for (int g = 0; g < kMaxGob; g++) {
-        const uint64_t v = (valid && g < n_gob) ? raw[g] : 0ull;
-        __builtin_memcpy(&sds[g], &v, sizeof(v));
-    }
-    /* MBA, MTYPE, MVD and CBP codes: code | len << 16 */
+    /* MBA, MTYPE, MVD and CBP codes: code | len << 16; GOB totals */
     __shared__ uint32_t t_mba[35], t_mtype[10], t_mvd[32], t_cbp[64];
-    {
+    __shared__ int gob_bits[kMaxGob];
+    if (wave == 0) {
         const uint32_t *g_mba = (const uint32_t *)c_mba, *g_mtype = (const uint32_t *)c_mtype,
                        *g_mvd = (const uint32_t *)c_mvd, *g_cbp = (const uint32_t *)c_cbp;
         const uint32_t a = g_mba[lane < 35 ? lane : 0], b = g_mtype[lane < 10 ? lane : 0],
@@ -214,16 +216,20 @@ __device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide
         if (lane < 10) t_mtype[lane] = b;
         if (lane < 32) t_mvd[lane] = c;
         t_cbp[lane] = e;
-        __syncthreads();
     }
+    __syncthreads();
     auto vcode = [](uint32_t e) { return e & 0xffffu; };
     auto vlen = [](uint32_t e) { return (int)((e >> 16) & 0xffu); };
-    int pos = kPictureHeaderBits; /* bit position in the frame */
-    int my_gob_pos = 0;           /* lane g: where GOB g's header starts */
+    /* per own GOB and lane: header codes, their length, the MB's bits, and
+     * the inclusive prefix of the bits over the GOB's MBs */
+    uint64_t hcs[kPer];
+    int hls[kPer], bs[kPer], incls[kPer];
 #pragma unroll
-    for (int g = 0; g < kMaxGob; g++) {
-        if (g >= n_gob) break;
-        const MbSide sd = sds[g];
+    for (int k = 0; k < kPer; k++) {
+        const int g = wave + k * kTailWaves;
+        uint64_t v = (valid && g < n_gob) ? raw[k] : 0ull;
+        MbSide sd;
+        __builtin_memcpy(&sd, &v, sizeof(v));
         const int flags = sd.flags;
         const bool coded = flags != 0;
         const uint64_t cm = __ballot(coded);
@@ -279,38 +285,63 @@ __device__ __forceinline__ int frame_bits_wave(const StreamDesc &d, const MbSide
                 }
             }
         }
-        const int gob_pos = pos;
-        if (lane == g) my_gob_pos = gob_pos;
         const int incl = wave_inclusive_sum(b, lane);
-        if (hdr && coded) {
-            const int gx = (g % gob_cols) * 11, gy = (g / gob_cols) * 3;
-            MbHdr h;
-            h.code = hc;
-            h.bitpos = gob_pos + kGobHeaderBits + incl - b;
-            h.len = (uint8_t)hl;
-            h.valid = 1;
-            h.bits = (uint16_t)b;
-            hdr[d.mb_base + (gy + lane / 11) * d.mb_w + gx + lane % 11] = h;
+        hcs[k] = hc;
+        hls[k] = hl;
+        bs[k] = b;
+        incls[k] = incl;
+        if (g < n_gob && lane == 0) gob_bits[g] = __builtin_amdgcn_readlane(incl, 63);
+    }
+    __syncthreads();
+    /* where each GOB starts: after the picture header and every earlier
+     * GOB (its header and its MBs) */
+    int start[kMaxGob];
+    int pos = kPictureHeaderBits;
+#pragma unroll
+    for (int g = 0; g < kMaxGob; g++) {
+        start[g] = pos;
+        if (g < n_gob) pos += kGobHeaderBits + gob_bits[g];
+    }
+    if (hdr) {
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int g = wave + k * kTailWaves;
+            const int gs = g < n_gob ? start[g] : 0;
+            if (g < n_gob && valid && bs[k] > 0) {
+                const int gx = (g % gob_cols) * 11, gy = (g / gob_cols) * 3;
+                MbHdr h;
+                h.code = hcs[k];
+                h.bitpos = gs + kGobHeaderBits + incls[k] - bs[k];
+                h.len = (uint8_t)hls[k];
+                h.valid = 1;
+                h.bits = (uint16_t)bs[k];
+                hdr[d.mb_base + (gy + lane / 11) * d.mb_w + gx + lane % 11] = h;
+            }
         }
-        pos = gob_pos + kGobHeaderBits + __builtin_amdgcn_readlane(incl, 63);
     }
     if (bits && ((pos + 31) >> 5) <= bits_words) {
         /* the frame's words start at zero; then the headers (the MB bits are
          * ORed in by the writer launch) */
         const int words = (pos + 31) >> 5;
-        for (int i = lane; i < words; i += 64) bits[i] = 0u;
+        for (int i = wave * 64 + lane; i < words; i += kTailWaves * 64) bits[i] = 0u;
         __syncthreads();
-        if (lane == 0) {
+        if (wave == 0 && lane == 0) {
             /* PSC, TR, PTYPE (split, document camera, freeze release, source
              * format, hi-res and spare bits as the reference sets them), PEI */
             const uint32_t ph = (16u << 12) | ((uint32_t)(d.frame_num & 31) << 7) | (1u << 4) | ((cif ? 1u : 0u) << 3) |
                                 (1u << 2) | (1u << 1);
             bits[0] = __builtin_bswap32(ph);
         }
-        if (lane < n_gob) {
-            /* GBSC, GN, GQUANT, GEI */
-            const uint32_t gn = cif ? lane + 1 : 2 * lane + 1;
-            or_bits(bits, my_gob_pos, (1u << 10) | (gn << 6) | ((uint32_t)picq << 1), kGobHeaderBits);
+        if (lane < kPer) {
+            /* GBSC, GN, GQUANT, GEI of this wave's GOBs, a lane each */
+            const int g = wave + lane * kTailWaves;
+            if (g < n_gob) {
+                const uint32_t gn = cif ? g + 1 : 2 * g + 1;
+                int gs = 0;
+#pragma unroll
+                for (int k = 0; k < kMaxGob; k++) gs = k == g ? start[k] : gs;
+                or_bits(bits, gs, (1u << 10) | (gn << 6) | ((uint32_t)picq << 1), kGobHeaderBits);
+            }
         }
     }
     return pos;
@@ -358,16 +389,17 @@ __device__ __forceinline__ void advance_one(StreamDesc &d, SeqState &q)
 /* the end of one stream's frame, all 64 lanes of one wave: denoise offsets
  * for the next frame, exact frame size, rate-control update, advance.  Reads
  * only what earlier launches wrote (statistics, side information). */
-__device__ __forceinline__ void frame_tail_wave(StreamDesc &d, SeqState &q, DenoiseState *dn, RcState &rc,
-                                                const MbSide *side, MbHdr *hdr, int32_t *bytes_out, int lane)
+__device__ __forceinline__ void frame_tail_group(StreamDesc &d, SeqState &q, DenoiseState *dn, RcState &rc,
+                                                 const MbSide *side, MbHdr *hdr, int32_t *bytes_out, int wave,
+                                                 int lane)
 {
     const int ci = dn->inter_count, ca = dn->intra_count;
-    denoise_update_lane(dn, lane, ci, ca);
+    if (wave == 0) denoise_update_lane(dn, lane, ci, ca);
     const bool write = hdr && q.bits;
-    const int bits = frame_bits_wave(d, side, rc.pic_quant, lane, write ? hdr : nullptr, write ? q.bits : nullptr,
-                                     q.bits_words);
+    const int bits = frame_bits_group(d, side, rc.pic_quant, wave, lane, write ? hdr : nullptr,
+                                      write ? q.bits : nullptr, q.bits_words);
     __syncthreads();
-    if (lane == 0) {
+    if (wave == 0 && lane == 0) {
         dn->inter_count = 0;
         dn->intra_count = 0;
         const int bytes = (bits + 7) >> 3;

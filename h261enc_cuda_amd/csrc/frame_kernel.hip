@@ -25,7 +25,7 @@ __global__ void rc_begin_kernel(RcState *rc_all, const StreamDesc *streams, int 
     rc.cost_accum = 0;
 }
 
-__global__ __launch_bounds__(64) void frame_bits_kernel(RcState *rc_all, const StreamDesc *streams, int n,
+__global__ __launch_bounds__(64 * fc::kTailWaves) void frame_bits_kernel(RcState *rc_all, const StreamDesc *streams, int n,
                                                          const MbSide *side, int32_t *bytes_out, const SeqState *seq,
                                                          MbHdr *hdr)
 {
@@ -34,8 +34,9 @@ __global__ __launch_bounds__(64) void frame_bits_kernel(RcState *rc_all, const S
     if (!(d.flags & kStreamActive)) return;
     RcState &rc = rc_all[s];
     const bool write = hdr && seq && seq[s].bits;
-    const int bits = fc::frame_bits_wave(d, side, rc.pic_quant, threadIdx.x, write ? hdr : nullptr,
-                                         write ? seq[s].bits : nullptr, write ? seq[s].bits_words : 0);
+    const int bits = fc::frame_bits_group(d, side, rc.pic_quant, threadIdx.x >> 6, threadIdx.x & 63,
+                                          write ? hdr : nullptr, write ? seq[s].bits : nullptr,
+                                          write ? seq[s].bits_words : 0);
     if (threadIdx.x == 0) {
         const int bytes = (bits + 7) >> 3;
         fc::rc_update_one(rc, d.frame_type, bytes);
@@ -59,15 +60,15 @@ __global__ __launch_bounds__(64) void denoise_update_kernel(DenoiseState *dn_all
     }
 }
 
-__global__ __launch_bounds__(64) void frame_tail_kernel(StreamDesc *streams, SeqState *seq, DenoiseState *dn_all,
+__global__ __launch_bounds__(64 * fc::kTailWaves) void frame_tail_kernel(StreamDesc *streams, SeqState *seq, DenoiseState *dn_all,
                                                          RcState *rc_all, const MbSide *side, MbHdr *hdr,
                                                          int32_t *bytes_out)
 {
     const int s = blockIdx.x;
     StreamDesc &d = streams[s];
     if (!(d.flags & kStreamActive)) return;
-    fc::frame_tail_wave(d, seq[s], dn_all + s, rc_all[s], side, hdr, bytes_out ? bytes_out + s : nullptr,
-                        threadIdx.x);
+    fc::frame_tail_group(d, seq[s], dn_all + s, rc_all[s], side, hdr, bytes_out ? bytes_out + s : nullptr,
+                         threadIdx.x >> 6, threadIdx.x & 63);
 }
 
 __global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
@@ -93,10 +94,10 @@ int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcSt
     }
     int32_t *no_bytes = nullptr;
     if (ev_start)
-        hipExtLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream,
+        hipExtLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64 * fc::kTailWaves), 0, (hipStream_t)stream,
                               (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0, streams, seq, dn, rc, side, hdr, no_bytes);
     else
-        hipLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, streams, seq, dn, rc,
+        hipLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64 * fc::kTailWaves), 0, (hipStream_t)stream, streams, seq, dn, rc,
                            side, hdr, no_bytes);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -121,7 +122,7 @@ int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, con
                       void *stream, const SeqState *seq, MbHdr *hdr)
 {
     if (n_streams <= 0) return 0;
-    hipLaunchKernelGGL(frame_bits_kernel, dim3(n_streams), dim3(64), 0, (hipStream_t)stream, rc, streams, n_streams,
+    hipLaunchKernelGGL(frame_bits_kernel, dim3(n_streams), dim3(64 * fc::kTailWaves), 0, (hipStream_t)stream, rc, streams, n_streams,
                        side, bytes_out, seq, hdr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
