@@ -104,13 +104,15 @@ __global__ void stream_set_kernel(StreamDesc *desc, SeqState *seq, int id, Strea
 
 /* re-packing of the MB index space: the new map, and each open stream's new
  * MB base (bases[id] < 0: slot not open) */
-__global__ void remap_kernel(StreamDesc *desc, int32_t *mb_stream, const int32_t *bases, const int32_t *map, int n,
-                             int max_streams)
+__global__ void remap_kernel(StreamDesc *desc, int32_t *mb_stream, const int32_t *bases)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    for (int k = i; k < n; k += stride) mb_stream[k] = map[k];
-    for (int k = i; k < max_streams; k += stride)
-        if (bases[k] >= 0) desc[k].mb_base = bases[k];
+    /* one workgroup per stream slot: an open stream's new range of the map
+     * and its descriptor's base */
+    const int s = blockIdx.x, base = bases[s];
+    if (base < 0) return;
+    const int n = desc[s].mb_w * desc[s].mb_h;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) mb_stream[base + k] = s;
+    if (threadIdx.x == 0) desc[s].mb_base = base;
 }
 
 /* a closed stream: inactive descriptor, MB range unmapped */
@@ -630,7 +632,12 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipMalloc((void **)&e->dn, sizeof(DenoiseState) * max_streams), "hipMalloc") &&
                 ok(hipMalloc((void **)&e->rc, sizeof(RcState) * max_streams), "hipMalloc") &&
                 ok(hipMemset(e->desc, 0, sizeof(StreamDesc) * max_streams), "hipMemset") &&
-                ok(hipMemset(e->seq, 0, sizeof(SeqState) * max_streams), "hipMemset");
+                ok(hipMemset(e->seq, 0, sizeof(SeqState) * max_streams), "hipMemset") &&
+                ok(hipHostMalloc((void **)&e->remap_host, sizeof(int32_t) * max_streams, hipHostMallocDefault),
+                   "hipHostMalloc") &&
+                ok(hipMalloc((void **)&e->remap_dev, sizeof(int32_t) * max_streams), "hipMalloc") &&
+                ok(hipEventCreateWithFlags(&e->remap_ev, hipEventDisableTiming), "event");
+    e->remap_cap = good ? (size_t)max_streams : 0;
     for (int r = 0; good && r < h261g_engine::kEvRing; r++)
         for (int i = 0; good && i < 8; i++) good = ok(hipEventCreate(&e->ev[r][i]), "event");
     if (!good) {
@@ -868,39 +875,36 @@ int h261g_engine_compact(h261g_engine_t *e)
 {
     if (!e) return H261G_ERR_ARG;
     if (flush_writer(e)) return H261G_ERR_DEVICE;
-    int n = 0;
-    for (int i = 0; i < e->max_streams; i++)
-        if (e->slots[i].open) n += e->slots[i].n_mb;
-    const size_t words = (size_t)e->max_streams + n;
+    /* only the new bases travel (a few KB): the remap kernel rebuilds the
+     * map from them; entries past the packed range are never read (launches
+     * cover mb_used MBs) */
+    const size_t words = (size_t)e->max_streams;
     if (e->remap_pending && !ok(hipEventSynchronize(e->remap_ev), "event")) return H261G_ERR_DEVICE;
     e->remap_pending = false;
     if (words > e->remap_cap) {
-        if (!ok(hipStreamSynchronize(e->stream), "sync")) return H261G_ERR_DEVICE;
         hipHostFree(e->remap_host);
         hipFree(e->remap_dev);
         e->remap_host = nullptr;
         e->remap_dev = nullptr;
         e->remap_cap = 0;
-        const size_t cap = words + words / 2;
-        if (!ok(hipHostMalloc((void **)&e->remap_host, cap * sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc") ||
-            !ok(hipMalloc((void **)&e->remap_dev, cap * sizeof(int32_t)), "hipMalloc"))
+        if (!ok(hipHostMalloc((void **)&e->remap_host, words * sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc") ||
+            !ok(hipMalloc((void **)&e->remap_dev, words * sizeof(int32_t)), "hipMalloc"))
             return H261G_ERR_MEMORY;
-        e->remap_cap = cap;
+        e->remap_cap = words;
     }
     if (!e->remap_ev && !ok(hipEventCreateWithFlags(&e->remap_ev, hipEventDisableTiming), "event"))
         return H261G_ERR_DEVICE;
-    int32_t *bases = e->remap_host, *map = e->remap_host + e->max_streams;
+    std::vector<int32_t> bases(words, -1);
     int used = 0;
     for (int i = 0; i < e->max_streams; i++) {
         Slot &s = e->slots[i];
-        bases[i] = -1;
         if (!s.open) continue;
         s.mb_base = used;
         s.desc0.mb_base = used;
         bases[i] = used;
-        for (int k = 0; k < s.n_mb; k++) map[used + k] = i;
         used += s.n_mb;
     }
+    memcpy(e->remap_host, bases.data(), words * sizeof(int32_t));
     e->mb_used = used;
     e->free_cif.clear();
     e->free_qcif.clear();
@@ -909,9 +913,7 @@ int h261g_engine_compact(h261g_engine_t *e)
         !ok(hipEventRecord(e->remap_ev, e->stream), "event"))
         return H261G_ERR_DEVICE;
     e->remap_pending = true;
-    const int threads = 256, blocks = (int)std::min<size_t>(1024, (words + threads - 1) / threads);
-    hipLaunchKernelGGL(remap_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, e->stream, e->desc, e->mb_stream,
-                       e->remap_dev, e->remap_dev + e->max_streams, used, e->max_streams);
+    hipLaunchKernelGGL(remap_kernel, dim3(e->max_streams), dim3(256), 0, e->stream, e->desc, e->mb_stream, e->remap_dev);
     if (!ok(hipGetLastError(), "remap")) return H261G_ERR_DEVICE;
     invalidate_graph(e);
     return H261G_OK;
