@@ -207,8 +207,11 @@ struct h261g_engine {
     hipEvent_t fetch_ev = nullptr;
     int mbstruct_cap = 0;
     /* graph of one step */
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    /* graphs of one step and of kGraphSteps steps: kernels inside a graph
+     * follow each other without a gap, a graph launch costs a few us */
+    static constexpr int kGraphSteps = 10;
+    hipGraph_t graph = nullptr, graph_k = nullptr;
+    hipGraphExec_t graph_exec = nullptr, graph_exec_k = nullptr;
     bool graph_valid = false;
     /* profiling */
     bool profile = false;
@@ -663,6 +666,8 @@ int h261g_engine_destroy(h261g_engine_t *e)
     if (e->remap_ev) (void)hipEventDestroy(e->remap_ev);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->graph_exec_k) (void)hipGraphExecDestroy(e->graph_exec_k);
+    if (e->graph_k) (void)hipGraphDestroy(e->graph_k);
     hipFree(e->desc);
     hipFree(e->seq);
     hipFree(e->dn);
@@ -1038,17 +1043,27 @@ int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
     if (!e->graph_valid) {
         if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
         if (e->graph) (void)hipGraphDestroy(e->graph);
-        e->graph_exec = nullptr;
-        e->graph = nullptr;
-        if (!ok(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "capture")) return H261G_ERR_DEVICE;
-        int rcode = enqueue_step(e, false);
-        hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
-        if (rcode || !ok(ce, "end capture")) return H261G_ERR_DEVICE;
-        if (!ok(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0), "instantiate"))
-            return H261G_ERR_DEVICE;
+        if (e->graph_exec_k) (void)hipGraphExecDestroy(e->graph_exec_k);
+        if (e->graph_k) (void)hipGraphDestroy(e->graph_k);
+        e->graph_exec = e->graph_exec_k = nullptr;
+        e->graph = e->graph_k = nullptr;
+        for (int k : {1, h261g_engine::kGraphSteps}) {
+            hipGraph_t &g = k == 1 ? e->graph : e->graph_k;
+            hipGraphExec_t &x = k == 1 ? e->graph_exec : e->graph_exec_k;
+            if (!ok(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "capture"))
+                return H261G_ERR_DEVICE;
+            int rcode = 0;
+            for (int i = 0; i < k && !rcode; i++) rcode = enqueue_step(e, false);
+            hipError_t ce = hipStreamEndCapture(e->stream, &g);
+            if (rcode || !ok(ce, "end capture")) return H261G_ERR_DEVICE;
+            if (!ok(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "instantiate")) return H261G_ERR_DEVICE;
+        }
         e->graph_valid = true;
     }
-    for (int i = 0; i < n_steps; i++)
+    int left = n_steps;
+    for (; left >= h261g_engine::kGraphSteps; left -= h261g_engine::kGraphSteps)
+        if (!ok(hipGraphLaunch(e->graph_exec_k, e->stream), "graph launch")) return H261G_ERR_DEVICE;
+    for (; left > 0; left--)
         if (!ok(hipGraphLaunch(e->graph_exec, e->stream), "graph launch")) return H261G_ERR_DEVICE;
     if (n_steps > 0)
         for (auto &s : e->slots) s.fresh = false;
