@@ -8,10 +8,15 @@
  *
  *   lanes 0-47 = (block b, row r):   MC / loop filter, residual, DCT rows
  *   LDS transpose
- *   lanes 0-47 = (block b, col c):   DCT columns, denoise, quantiser, CBP,
- *                                    dequantiser, IDCT columns
+ *   lanes 0-47 = (block b, col c):   DCT columns, denoise, quantiser, CBP
+ *   lanes 0-47 = (block b, row r):   an uncoded block's reconstruction is
+ *                                    its prediction (all-zero levels give a
+ *                                    zero IDCT, mbgpu:966-976); a coded
+ *                                    block keeps its prediction in LDS
+ * and then, for the coded blocks only, packed 8 to a pass:
+ *   lane (slot, col c):              dequantiser, IDCT columns
  *   LDS transpose
- *   lanes 0-47 = (block b, row r):   IDCT rows, reconstruction
+ *   lane (slot, row r):              IDCT rows, reconstruction
  *
  * Arithmetic follows the reference op for op (compiled with
  * -ffp-contract=off and '#pragma clang fp contract(off)'): products with an
@@ -226,15 +231,22 @@ __device__ __forceinline__ void flush_stats(const MbLaunch &p, int s, int32_t (*
 
 /* One wave codes groups of G consecutive MBs (G = 1: one pass of 6 blocks,
  * lanes 48-63 idle; G = 4: 24 blocks in three full passes of 8).  Per pass:
- * phase 1 lane (block, row), phase 2 lane (block, column), phase 3 lane
- * (block, row); then per MB of the group the CBP, bit count and outputs. */
+ * phase 1 lane (block, row), phase 2 lane (block, column), then lane (block,
+ * row) writes an uncoded block's reconstruction or keeps a coded block's
+ * prediction.  The group's coded blocks then go through the dequantiser,
+ * IDCT and reconstruction 8 at a time (at the batched bitrates about 9% of
+ * inter blocks are coded, so mostly one pass or none).  Last, per MB of the
+ * group, the CBP, bit count and outputs. */
 template <int G>
 __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 {
     constexpr int kBlocks = 6 * G;
     constexpr int kPasses = (kBlocks + 7) / 8;
     __shared__ float tf[8 * kBlkPitch];
-    __shared__ __align__(16) int16_t lv[kBlocks * 64];
+    __shared__ __align__(16) int8_t lv[kBlocks * 64]; /* levels; an intra DC (1..255) as uint8 */
+    __shared__ uint2 pred_lds[kBlocks * 8];          /* coded blocks: prediction rows */
+    __shared__ uint4 desc[kBlocks];                  /* coded blocks: reconstruction address, stride,
+                                                        qmul | qadd << 8 | intra << 16 */
     __shared__ int32_t stat[2][2][64]; /* [stream slot][intra, inter][coefficient] */
     __shared__ int32_t scount[2][2];
     __shared__ __align__(16) uint32_t tlen32[kTlenWords];
@@ -358,7 +370,6 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             __syncthreads();
 
             /* ---- phase 2: lane (block, column c) ---- */
-            int lvl[8];
             int blk_accum = 0;
             if (L.act) {
                 const int c = rc;
@@ -405,8 +416,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                         const int al = l < 0 ? -l : l;
                         blk_accum += al > 1 ? 100 : al;
                     }
-                    lvl[r] = l;
-                    lv[bi * 64 + i] = (int16_t)l;
+                    lv[bi * 64 + i] = (int8_t)l;
                 }
             }
             /* sum the 8 columns of the block */
@@ -420,20 +430,55 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             for (int k = 0; k < 8; k++)
                 if ((coded_mask >> (8 * k)) & 1) coded_bits |= 1u << (pass * 8 + k);
 
+            /* lane (block, row rc) holds row rc of the block's prediction */
             if (L.act) {
+                const StreamDesc &d = p.streams[L.s];
+                uint8_t *dst = d.rec[L.plane] + (size_t)L.py * L.stride + L.px;
+                if (!coded) {
+                    *(uint2 *)(dst + (size_t)rc * L.stride) = make_uint2(predw[0], predw[1]);
+                } else {
+                    pred_lds[bi * 8 + rc] = make_uint2(predw[0], predw[1]);
+                    if (rc == 0) {
+                        const int qadd = (L.quant - 1) | 1, qmul = L.quant * 2;
+                        desc[bi] = make_uint4((uint32_t)(uintptr_t)dst, (uint32_t)((uintptr_t)dst >> 32),
+                                              (uint32_t)L.stride, (uint32_t)(qmul | qadd << 8 | (L.intra ? 1 << 16 : 0)));
+                    }
+                }
+            }
+            __syncthreads(); /* tf is reused by the next pass */
+        }
+
+        /* ---- the group's coded blocks, 8 per pass: slot k8 takes the
+         * (8 cp + k8)-th set bit of coded_bits ---- */
+        const int n_coded = __builtin_popcount(coded_bits);
+        uint32_t rest = coded_bits;
+#pragma unroll 1
+        for (int cp = 0; cp * 8 < n_coded; cp++) {
+            int bi = -1;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int b = rest ? __builtin_ctz(rest) : -1;
+                bi = k8 == j ? b : bi;
+                rest &= rest - 1;
+            }
+            const bool act = bi >= 0;
+            const uint4 de = desc[act ? bi : 0];
+            const bool intra = (de.w >> 16) & 1;
+            const int qmul = (int)(de.w & 0xff), qadd = (int)((de.w >> 8) & 0xff);
+            if (act) {
                 const int c = rc;
-                /* dequantiser (mbgpu:763-814); an uncoded block is all zero */
-                const int qadd = coded ? ((L.quant - 1) | 1) : 0, qmul = coded ? L.quant * 2 : 0;
+                /* dequantiser (mbgpu:763-814) */
                 float tv[8], o8[8];
 #pragma unroll
                 for (int r = 0; r < 8; r++) {
                     const int i = c + 8 * r;
-                    const int l = lvl[r];
+                    const int raw = lv[bi * 64 + i];
                     int dq;
-                    if (L.intra && i == 0) {
+                    if (intra && i == 0) {
+                        const int l = raw & 0xff;
                         dq = l == 255 ? 1024 : l * 8;
                     } else {
-                        int x = l * qmul;
+                        int x = raw * qmul;
                         if (x > 0)
                             x += qadd;
                         else if (x < 0)
@@ -446,20 +491,19 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                 }
                 /* IDCT column pass (mbgpu:851-888) */
                 idct_core(tv, o8);
-                __builtin_amdgcn_s_barrier(); /* all phase-2 reads of tf are done */
 #pragma unroll
                 for (int r = 0; r < 8; r++) tf[k8 * kBlkPitch + r * 8 + c] = o8[r];
-            } else {
-                __builtin_amdgcn_s_barrier();
             }
             __syncthreads();
 
-            /* ---- phase 3: lane (block, row): IDCT rows, reconstruction ---- */
-            if (L.act) {
+            /* lane (slot, row rc): IDCT rows, reconstruction */
+            if (act) {
                 float v[8], o8[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) v[k] = tf[k8 * kBlkPitch + rc * 8 + k];
                 idct_core(v, o8);
+                const uint2 pw = pred_lds[bi * 8 + rc];
+                const uint32_t predw[2] = {pw.x, pw.y};
                 uint32_t w[2] = {0, 0};
 #pragma unroll
                 for (int x = 0; x < 8; x++) {
@@ -468,10 +512,10 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     const int val = clampi(pv + rres, 0, 255);
                     w[x >> 2] |= (uint32_t)val << (8 * (x & 3));
                 }
-                const StreamDesc &d = p.streams[L.s];
-                *(uint2 *)(d.rec[L.plane] + (size_t)(L.py + rc) * L.stride + L.px) = make_uint2(w[0], w[1]);
+                uint8_t *dst = (uint8_t *)(((uintptr_t)de.y << 32) | de.x);
+                *(uint2 *)(dst + (size_t)rc * de.z) = make_uint2(w[0], w[1]);
             }
-            __syncthreads(); /* tf is reused by the next pass */
+            __syncthreads(); /* tf and slots are reused by the next pass */
         }
 
         /* ---- per MB of the group: CBP, bit count of the CBP + TCOEFF
@@ -493,7 +537,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 #pragma unroll
             for (int k = 0; k < 6; k++)
                 if ((coded_bits >> (6 * mi + k)) & 1) cbp |= kCbpBit[k];
-            const int16_t *mlv = lv + 6 * mi * 64;
+            const int8_t *mlv = lv + 6 * mi * 64;
 
             int bits = 0;
             if (p.side && (intra || cbp)) {
@@ -501,7 +545,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                 for (int k = 0; k < 6; k++) {
                     const bool blk_written = intra || (cbp & kCbpBit[k]);
                     if (!blk_written) continue;
-                    const int l = mlv[k * 64 + zz];
+                    const int l = mlv[k * 64 + zz]; /* an intra DC reads as non-zero either way */
                     const uint64_t nz = __ballot(l != 0);
                     if (l != 0) {
                         const uint64_t below = lane ? (nz & ((~0ull) >> (64 - lane))) : 0ull;
@@ -533,16 +577,14 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
                     const int w = lane + 64 * j;
-                    if (w < 96) {
-                        const uint2 v = *(const uint2 *)&mlv[4 * w];
-                        out[w] = (v.x & 0xffu) | ((v.x >> 8) & 0xff00u) | ((v.y & 0xffu) << 16) | ((v.y >> 16) << 24);
-                    }
+                    if (w < 96) out[w] = ((const uint32_t *)mlv)[w];
                 }
             }
             if (p.mb_struct) {
                 h261_macroblock_t *o = p.mb_struct + g;
 #pragma unroll
-                for (int k = 0; k < 6; k++) o->rgi_tcoeff[k][lane] = mlv[k * 64 + lane];
+                for (int k = 0; k < 6; k++)
+                    o->rgi_tcoeff[k][lane] = intra && lane == 0 ? (int)(uint8_t)mlv[k * 64] : (int)mlv[k * 64 + lane];
                 if (lane == 0) {
                     o->i_macroblock_address = 0;
                     o->i_macroblock_type = md.type;

@@ -31,6 +31,8 @@ namespace h261g {
 namespace {
 
 constexpr int kWinRows = 46;  /* 16 + 2*15 */
+/* word 3 of a gfx9 buffer resource: 32-bit data format, raw addressing */
+constexpr int kBufferDword3 = 0x00020000;
 constexpr int kWinDw = 12;    /* 48 bytes per row: frame cols bx-16 .. bx+31 */
 /* LDS dwords per window row: 15 for the two-stage search (with the lanes of
  * a coarse or refine pass spread over rows 2-3 apart, a 15-dword pitch puts
@@ -40,6 +42,20 @@ template <int kVariant>
 constexpr int win_pitch()
 {
     return kVariant == 1 ? kWinDw : 15;
+}
+/* The two-stage window has two spare rows above and below and one spare
+ * dword in front: a refine candidate outside the limits (up to 17 px from
+ * the MB) then still reads LDS inside the array; its cost is replaced by
+ * 65535. win = winmem + win_base. */
+template <int kVariant>
+constexpr int win_base()
+{
+    return kVariant == 1 ? 0 : 2 * win_pitch<kVariant>() + 1;
+}
+template <int kVariant>
+constexpr int win_alloc()
+{
+    return kVariant == 1 ? kWinRows * kWinDw : (win_base<kVariant>() + (kWinRows + 2) * win_pitch<kVariant>() + 3) & ~3;
 }
 
 /* minimum over the wave: quads, half rows and rows by DPP, rows by readlane */
@@ -80,10 +96,94 @@ __device__ __forceinline__ void unmorton(uint32_t k, int &x, int &y)
     }
 }
 
-__device__ __forceinline__ int coarse_vec(int i)
+__host__ __device__ constexpr int coarse_vec(int i)
 {
     /* {-14,-11,-8,-5,-2,0,2,5,8,11,14} */
     return i < 5 ? 3 * i - 14 : (i == 5 ? 0 : 3 * i - 16);
+}
+
+constexpr uint32_t morton_c(uint32_t x, uint32_t y)
+{
+    uint32_t k = 0;
+    for (int b = 0; b < 4; b++) k |= (((x >> b) & 1u) << (2 * b)) | (((y >> b) & 1u) << (2 * b + 1));
+    return k;
+}
+
+/* Per-lane candidate tables of the two-stage search (window pitch 15).
+ * A candidate (vx, vy) starts at LDS byte 4 * 15 * (vy + 15) + vx + 16 of
+ * the window: its dword is that >> 2, its byte shift that & 3.
+ *   coarse: slot i of lane L takes coarse candidate L + 64 i (lanes 57-63 of
+ *           slot 1 repeat candidate 120 and are masked off): byte address |
+ *           Z-order key << 16;
+ *   refine: lane L < 49 takes (tx, ty) = (L % 7, L / 7) (lanes 49-63 repeat
+ *           lane 0 and are masked off): 60 * ty + tx | Z-order key << 16,
+ *           added to the winner's byte address. */
+constexpr int kTwoStagePitch = 15;
+struct CoarseLanes {
+    uint32_t e[2][64];
+};
+constexpr CoarseLanes make_coarse_lanes()
+{
+    CoarseLanes t{};
+    for (int i = 0; i < 2; i++)
+        for (int l = 0; l < 64; l++) {
+            const int ci = l + 64 * i < 121 ? l + 64 * i : 120;
+            const int tx = ci % 11, ty = ci / 11;
+            const int a = 4 * kTwoStagePitch * (coarse_vec(ty) + 15) + coarse_vec(tx) + 16;
+            t.e[i][l] = (uint32_t)a | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
+        }
+    return t;
+}
+struct RefineLanes {
+    uint32_t e[64];
+};
+constexpr RefineLanes make_refine_lanes()
+{
+    RefineLanes t{};
+    for (int l = 0; l < 64; l++) {
+        const int c = l < 49 ? l : 0;
+        const int tx = c % 7, ty = c / 7;
+        t.e[l] = (uint32_t)(4 * kTwoStagePitch * ty + tx) | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
+    }
+    return t;
+}
+__constant__ CoarseLanes c_coarse = make_coarse_lanes();
+__constant__ RefineLanes c_refine = make_refine_lanes();
+
+/* lane masks of the in-limit candidates: bit ci = tx + 11 ty of the coarse
+ * grid (two 64-bit halves), bit tx + 7 ty of the refine square; wave-uniform
+ * values, so this is scalar work */
+__device__ __forceinline__ void coarse_masks(int lx0, int lx1, int ly0, int ly1, uint64_t &m0, uint64_t &m1)
+{
+    uint32_t xm = 0, ym = 0;
+#pragma unroll
+    for (int t = 0; t < 11; t++) {
+        const int v = coarse_vec(t);
+        xm |= (uint32_t)(v >= lx0 && v <= lx1) << t;
+        ym |= (uint32_t)(v >= ly0 && v <= ly1) << t;
+    }
+    m0 = 0;
+    m1 = 0;
+#pragma unroll
+    for (int ty = 0; ty < 11; ty++) {
+        const uint64_t r = ((ym >> ty) & 1u) ? (uint64_t)xm : 0ull;
+        const int sh = 11 * ty;
+        if (sh < 64) m0 |= r << sh;
+        if (sh + 11 > 64) m1 |= sh >= 64 ? r << (sh - 64) : r >> (64 - sh);
+    }
+    m1 &= (1ull << 57) - 1; /* candidates 64..120 */
+}
+
+__device__ __forceinline__ uint64_t refine_mask(int sx, int sy, int lx0, int lx1, int ly0, int ly1)
+{
+    const int x0 = max(lx0 - sx + 3, 0), x1 = min(lx1 - sx + 3, 6);
+    const int y0 = max(ly0 - sy + 3, 0), y1 = min(ly1 - sy + 3, 6);
+    const uint32_t xm = x1 >= x0 ? (2u << x1) - (1u << x0) : 0u;
+    uint64_t m = 0;
+#pragma unroll
+    for (int ty = 0; ty < 7; ty++)
+        if (ty >= y0 && ty <= y1) m |= (uint64_t)xm << (7 * ty);
+    return m;
 }
 
 /* one row of a candidate: window dwords o..o+4 realigned by sh bytes against
@@ -117,17 +217,6 @@ __device__ __forceinline__ void sad16(const uint32_t *win, const uint8_t *cur_mb
 #pragma unroll
         for (int i = 0; i < NS; i++) acc[i] = sad_row(win + off[i] + k * kPitch, sh[i], c0, c1, c2, c3, acc[i]);
     }
-}
-
-/* window offset and byte shift of candidate (vx, vy); candidates outside
- * the limits are evaluated at (0, 0) (always inside the window) and their
- * cost replaced by 65535 */
-template <int kPitch>
-__device__ __forceinline__ void cand_addr(int vx, int vy, bool valid, int &off, int &sh)
-{
-    const int x = valid ? vx : 0, y = valid ? vy : 0;
-    off = (y + 15) * kPitch + ((x + 16) >> 2);
-    sh = (x + 16) & 3;
 }
 
 typedef short short2v __attribute__((ext_vector_type(2)));
@@ -216,7 +305,8 @@ template <int kVariant>
 __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
 {
     constexpr int kPitch = win_pitch<kVariant>();
-    __shared__ __align__(16) uint32_t win[kWinRows * kPitch];
+    __shared__ __align__(16) uint32_t winmem[win_alloc<kVariant>()];
+    uint32_t *const win = winmem + win_base<kVariant>();
     __shared__ __align__(16) uint32_t curw[kVariant == kMeSurface ? 64 : 4];
     __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
 
@@ -239,24 +329,37 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
     const int ly0 = max(by - 15, 0) - by, ly1 = min(by + 15, (d.mb_h - 1) * 16) - by;
 
     /* stage the window: all loads issued before any wait.  Lane (c4, rr) of
-     * lanes 0-59 loads dword c4 of rows rr, rr + 5, ...; window dwords
-     * outside the frame come from a clamped in-frame address (no in-limit
-     * candidate or filter tap reads them), rows past the window are loaded
-     * from a clamped row and dropped. */
+     * lanes 0-59 loads dword c4 of rows rr, rr + 5, ... (row 45 only for
+     * rr = 0); window dwords outside the frame come from a clamped in-frame
+     * address (no in-limit candidate or filter tap reads them).  When all 46
+     * rows lie inside the frame, each staged row has a scalar base and the
+     * lane a 32-bit offset. */
     constexpr int kStageIters = (kWinRows + 4) / 5; /* 10 */
-    const int c4 = lane % kWinDw, rr = lane / kWinDw;
+    const int c4 = lane % kWinDw, rr = min(lane / kWinDw, 4);
     const int fx = min(max(bx - 16 + 4 * c4, 0), W - 4);
     uint32_t staged[kStageIters];
+    /* buffer loads over the luma plane: a 32-bit lane offset, and in the
+     * interior case a scalar row offset (addresses stay in the plane, so its
+     * range check never applies) */
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)ref, 0, W * H, kBufferDword3);
+    if (by >= 15 && by + 30 < H) {
+        const uint32_t lo = (uint32_t)(rr * W + fx);
 #pragma unroll
-    for (int t = 0; t < kStageIters; t++) {
-        const int fy = min(max(by - 15 + rr + 5 * t, 0), H - 1);
-        staged[t] = *(const uint32_t *)(ref + (size_t)fy * W + fx);
+        for (int t = 0; t < kStageIters - 1; t++)
+            staged[t] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, lo, (by - 15 + 5 * t) * W, 0);
+        staged[kStageIters - 1] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)fx, (by + 30) * W, 0);
+    } else {
+#pragma unroll
+        for (int t = 0; t < kStageIters; t++) {
+            const int fy = min(max(by - 15 + (t < kStageIters - 1 ? rr : 0) + 5 * t, 0), H - 1);
+            staged[t] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)(fy * W + fx), 0, 0);
+        }
     }
     const uint32_t cw = *(const uint32_t *)(cur + (size_t)(by + (lane >> 2)) * W + bx + 4 * (lane & 3));
+    if (lane < 5 * kWinDw) {
 #pragma unroll
-    for (int t = 0; t < kStageIters; t++) {
-        const int r = rr + 5 * t;
-        if (lane < 5 * kWinDw && r < kWinRows) win[r * kPitch + c4] = staged[t];
+        for (int t = 0; t < kStageIters - 1; t++) win[(rr + 5 * t) * kPitch + c4] = staged[t];
+        if (rr == 0) win[(kWinRows - 1) * kPitch + c4] = staged[kStageIters - 1];
     }
     if (kVariant == kMeSurface) curw[lane] = cw;
     __syncthreads();
@@ -283,25 +386,27 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
                 best = key < best ? key : best;
             }
         } else {
-            int tx[2], ty[2], off[2], sh[2];
-            bool valid[2];
+            /* every coarse candidate lies inside the window, so each is
+             * evaluated where it is; the mask bit of an out-of-limit one
+             * replaces its cost by 65535 (lanes 57-63 of slot 1 are off) */
+            static_assert(kVariant == kMeSurface || kPitch == kTwoStagePitch, "lane tables assume the two-stage pitch");
+            uint64_t m[2];
+            coarse_masks(lx0, lx1, ly0, ly1, m[0], m[1]);
+            int off[2], sh[2];
+            uint32_t e[2];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                const int ci = min(lane + 64 * i, 120);
-                tx[i] = ci % 11;
-                ty[i] = ci / 11;
-                const int vx = coarse_vec(tx[i]), vy = coarse_vec(ty[i]);
-                valid[i] = vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1 && lane + 64 * i < 121;
-                cand_addr<kPitch>(vx, vy, valid[i], off[i], sh[i]);
+                e[i] = c_coarse.e[i][lane];
+                off[i] = (int)(e[i] & 0xffffu) >> 2;
+                sh[i] = (int)(e[i] & 3u);
             }
             uint32_t acc[2];
             sad16<2, kPitch>(win, cur_mb, W, off, sh, acc);
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                const uint32_t cost = valid[i] ? acc[i] : 65535u;
-                const uint32_t key = (cost << 8) | morton((uint32_t)tx[i], (uint32_t)ty[i]);
-                /* lanes 57-63 hold a duplicate of candidate 120 in slot 1 */
-                if (lane + 64 * i < 121) best = key < best ? key : best;
+                const uint32_t cost = __builtin_amdgcn_inverse_ballot_w64(m[i]) ? acc[i] : 65535u;
+                const uint32_t key = (cost << 8) | (e[i] >> 16);
+                best = key < best ? key : best;
             }
         }
         best = wave_min_u32(best);
@@ -309,22 +414,24 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
         unmorton(best & 0xffu, wtx, wty);
         const int sx = coarse_vec(wtx), sy = coarse_vec(wty);
 
-        /* stage 2: 7x7 refine around the coarse winner */
-        {
+        /* stage 2: 7x7 refine around the coarse winner (always in limits,
+         * since (0, 0) is and no SAD reaches 65535) */
+        if (kVariant == kMeSurface) {
             const int tx = (lane < 49 ? lane : 0) % 7, ty = (lane < 49 ? lane : 0) / 7;
-            const int vx = sx + tx - 3, vy = sy + ty - 3;
-            uint32_t cost;
-            if (kVariant == kMeSurface) {
-                cost = cost_of(vx, vy);
-            } else {
-                const bool valid = vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1;
-                int off[1], sh[1];
-                cand_addr<kPitch>(vx, vy, valid, off[0], sh[0]);
-                uint32_t acc[1];
-                sad16<1, kPitch>(win, cur_mb, W, off, sh, acc);
-                cost = valid ? acc[0] : 65535u;
-            }
+            const uint32_t cost = cost_of(sx + tx - 3, sy + ty - 3);
             best = lane < 49 ? (cost << 8) | morton((uint32_t)tx, (uint32_t)ty) : 0xffffffffu;
+        } else {
+            /* lane (tx, ty) starts at the byte address of (sx - 3, sy - 3)
+             * plus 60 ty + tx; out-of-limit candidates (at most 17 px out)
+             * read the window's spare rows and are masked off */
+            const uint64_t m2 = refine_mask(sx, sy, lx0, lx1, ly0, ly1);
+            const uint32_t e2 = c_refine.e[lane];
+            const int a = 4 * kPitch * (sy + 12) + (sx + 13) + (int)(e2 & 0xffffu);
+            int off[1] = {a >> 2}, sh[1] = {a & 3};
+            uint32_t acc[1];
+            sad16<1, kPitch>(win, cur_mb, W, off, sh, acc);
+            const uint32_t cost = __builtin_amdgcn_inverse_ballot_w64(m2) ? acc[0] : 65535u;
+            best = (cost << 8) | (e2 >> 16);
         }
         best = wave_min_u32(best);
         unmorton(best & 0xffu, wtx, wty);
