@@ -112,28 +112,84 @@ constexpr uint32_t morton_c(uint32_t x, uint32_t y)
 /* Per-lane candidate tables of the two-stage search (window pitch 15).
  * A candidate (vx, vy) starts at LDS byte 4 * 15 * (vy + 15) + vx + 16 of
  * the window: its dword is that >> 2, its byte shift that & 3.
- *   coarse: slot i of lane L takes coarse candidate L + 64 i (lanes 57-63 of
- *           slot 1 repeat candidate 120 and are masked off): byte address |
- *           Z-order key << 16;
- *   refine: lane L < 49 takes (tx, ty) = (L % 7, L / 7) (lanes 49-63 repeat
- *           lane 0 and are masked off): 60 * ty + tx | Z-order key << 16,
- *           added to the winner's byte address. */
+ *
+ * Lane layout.  LDS serves a wave's dword reads a half-wave (32 lanes) at a
+ * time over 32 banks; lanes reading different dwords of one bank wait for
+ * each other.  All lanes of a read add the same row and dword to their
+ * candidate's start, so a half-wave reads without conflict when its
+ * candidates' distinct start dwords lie in distinct banks.
+ *   coarse: kCoarseOrder[2 i + h][j] is the candidate (tx + 11 ty) of lane
+ *           32 h + j in slot i, an assignment of the 121 candidates found by
+ *           a search for that property (88 distinct start dwords in 4 x 32
+ *           bank slots; checked below).  Spare lanes repeat the first
+ *           candidate of their half-wave (the same dword, so a broadcast)
+ *           and are masked off;
+ *   refine: the 7 x 7 rows ty go to lanes 7 j + tx of half-wave 0 in the
+ *           order 0, 1, 4, 5 and of half-wave 1 in the order 2, 3, 6: rows
+ *           15 dwords apart then never meet in a bank, whatever the winner's
+ *           byte alignment.  Spare lanes repeat the first lane of their
+ *           half-wave and are masked off.
+ * Entries: byte address (coarse) or 60 ty + tx added to the winner's byte
+ * address (refine) | Z-order key << 16. */
 constexpr int kTwoStagePitch = 15;
+constexpr int8_t kCoarseOrder[4][32] = {
+    {0, 5, 6, 7, 13, 14, 16, 17, 24, 25, 26, 27, 28, 29, 32, 49, 50, 52, 53, 57, 58, 73, 74, 75, 76, 88, 89, 104, 105, -1, -1, -1},
+    {8, 9, 10, 11, 34, 35, 36, 43, 54, 55, 56, 60, 61, 62, 66, 67, 71, 72, 78, 85, 86, 100, 101, 102, 103, 107, 108, 110, 118, 119, 120, -1},
+    {4, 12, 18, 19, 20, 21, 23, 30, 31, 37, 48, 59, 63, 64, 65, 77, 82, 83, 84, 87, 90, 91, 92, 96, 97, 112, 113, 114, 117, -1, -1, -1},
+    {1, 2, 3, 15, 22, 33, 38, 39, 40, 41, 42, 44, 45, 46, 47, 51, 68, 69, 70, 79, 80, 81, 93, 94, 95, 98, 99, 106, 109, 111, 115, 116},
+};
+constexpr int kRefineRowBase[7] = {0, 7, 32, 39, 14, 21, 46}; /* first lane of refine row ty */
+
+constexpr int coarse_start(int ci)
+{
+    return 4 * kTwoStagePitch * (coarse_vec(ci / 11) + 15) + coarse_vec(ci % 11) + 16;
+}
+constexpr int coarse_at(int i, int l) /* candidate of lane l in slot i, spare lanes resolved */
+{
+    const int c = kCoarseOrder[2 * i + l / 32][l % 32];
+    return c >= 0 ? c : kCoarseOrder[2 * i + l / 32][0];
+}
 struct CoarseLanes {
     uint32_t e[2][64];
+    uint64_t row_below[2][12]; /* lanes of slot i whose candidate has ty < t */
+    uint64_t col_below[2][12]; /* ... tx < t */
 };
 constexpr CoarseLanes make_coarse_lanes()
 {
     CoarseLanes t{};
     for (int i = 0; i < 2; i++)
         for (int l = 0; l < 64; l++) {
-            const int ci = l + 64 * i < 121 ? l + 64 * i : 120;
-            const int tx = ci % 11, ty = ci / 11;
-            const int a = 4 * kTwoStagePitch * (coarse_vec(ty) + 15) + coarse_vec(tx) + 16;
-            t.e[i][l] = (uint32_t)a | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
+            const int ci = coarse_at(i, l), tx = ci % 11, ty = ci / 11;
+            t.e[i][l] = (uint32_t)coarse_start(ci) | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
+            if (kCoarseOrder[2 * i + l / 32][l % 32] < 0) continue;
+            for (int b = ty + 1; b < 12; b++) t.row_below[i][b] |= 1ull << l;
+            for (int b = tx + 1; b < 12; b++) t.col_below[i][b] |= 1ull << l;
         }
     return t;
 }
+/* every candidate in exactly one lane; distinct start dwords of a half-wave
+ * in distinct banks */
+constexpr bool coarse_order_ok()
+{
+    int seen[121] = {};
+    for (int k = 0; k < 4; k++)
+        for (int j = 0; j < 32; j++) {
+            const int c = kCoarseOrder[k][j];
+            if (c < -1 || c > 120) return false;
+            if (c >= 0) seen[c]++;
+            for (int j2 = 0; j2 < j; j2++) {
+                const int c2 = kCoarseOrder[k][j2];
+                if (c < 0 || c2 < 0) continue;
+                const int a = coarse_start(c) >> 2, a2 = coarse_start(c2) >> 2;
+                if (a != a2 && a % 32 == a2 % 32) return false;
+            }
+        }
+    for (int c = 0; c < 121; c++)
+        if (seen[c] != 1) return false;
+    return true;
+}
+static_assert(coarse_order_ok(), "coarse lane layout");
+
 struct RefineLanes {
     uint32_t e[64];
 };
@@ -141,18 +197,40 @@ constexpr RefineLanes make_refine_lanes()
 {
     RefineLanes t{};
     for (int l = 0; l < 64; l++) {
-        const int c = l < 49 ? l : 0;
-        const int tx = c % 7, ty = c / 7;
+        int tx = 0, ty = l < 32 ? 0 : 2; /* spare lanes: the first lane of the half-wave */
+        for (int r = 0; r < 7; r++)
+            if (l >= kRefineRowBase[r] && l < kRefineRowBase[r] + 7) {
+                tx = l - kRefineRowBase[r];
+                ty = r;
+            }
         t.e[l] = (uint32_t)(4 * kTwoStagePitch * ty + tx) | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
     }
     return t;
 }
+/* rows of a half-wave: start dwords 15 ty + (a + tx) / 4, a = 0..3, in
+ * distinct banks for distinct dwords */
+constexpr bool refine_rows_ok()
+{
+    for (int a = 0; a < 4; a++)
+        for (int r = 0; r < 7; r++)
+            for (int r2 = 0; r2 < 7; r2++)
+                for (int x = 0; x < 7; x++)
+                    for (int x2 = 0; x2 < 7; x2++) {
+                        if ((kRefineRowBase[r] < 32) != (kRefineRowBase[r2] < 32)) continue;
+                        const int d = kTwoStagePitch * r + (a + x) / 4, d2 = kTwoStagePitch * r2 + (a + x2) / 4;
+                        if (d != d2 && d % 32 == d2 % 32) return false;
+                    }
+    return kRefineRowBase[3] + 7 <= 64;
+}
+static_assert(refine_rows_ok(), "refine lane layout");
+
 __constant__ CoarseLanes c_coarse = make_coarse_lanes();
 __constant__ RefineLanes c_refine = make_refine_lanes();
 
-/* lane masks of the in-limit candidates: bit ci = tx + 11 ty of the coarse
- * grid (two 64-bit halves), bit tx + 7 ty of the refine square; wave-uniform
- * values, so this is scalar work */
+/* lane masks of the in-limit candidates (wave-uniform, so scalar work).
+ * The valid coarse candidates are a product of a tx range and a ty range
+ * (coarse_vec is increasing), so slot i's mask is the lanes with ty in
+ * [y0, y1] and tx in [x0, x1]. */
 __device__ __forceinline__ void coarse_masks(int lx0, int lx1, int ly0, int ly1, uint64_t &m0, uint64_t &m1)
 {
     uint32_t xm = 0, ym = 0;
@@ -162,16 +240,13 @@ __device__ __forceinline__ void coarse_masks(int lx0, int lx1, int ly0, int ly1,
         xm |= (uint32_t)(v >= lx0 && v <= lx1) << t;
         ym |= (uint32_t)(v >= ly0 && v <= ly1) << t;
     }
-    m0 = 0;
-    m1 = 0;
-#pragma unroll
-    for (int ty = 0; ty < 11; ty++) {
-        const uint64_t r = ((ym >> ty) & 1u) ? (uint64_t)xm : 0ull;
-        const int sh = 11 * ty;
-        if (sh < 64) m0 |= r << sh;
-        if (sh + 11 > 64) m1 |= sh >= 64 ? r << (sh - 64) : r >> (64 - sh);
-    }
-    m1 &= (1ull << 57) - 1; /* candidates 64..120 */
+    /* (0, 0) is always in the limits, so neither range is empty */
+    const int x0 = __builtin_ctz(xm), x1 = 31 - __builtin_clz(xm);
+    const int y0 = __builtin_ctz(ym), y1 = 31 - __builtin_clz(ym);
+    m0 = (c_coarse.row_below[0][y1 + 1] & ~c_coarse.row_below[0][y0]) &
+         (c_coarse.col_below[0][x1 + 1] & ~c_coarse.col_below[0][x0]);
+    m1 = (c_coarse.row_below[1][y1 + 1] & ~c_coarse.row_below[1][y0]) &
+         (c_coarse.col_below[1][x1 + 1] & ~c_coarse.col_below[1][x0]);
 }
 
 __device__ __forceinline__ uint64_t refine_mask(int sx, int sy, int lx0, int lx1, int ly0, int ly1)
@@ -182,7 +257,7 @@ __device__ __forceinline__ uint64_t refine_mask(int sx, int sy, int lx0, int lx1
     uint64_t m = 0;
 #pragma unroll
     for (int ty = 0; ty < 7; ty++)
-        if (ty >= y0 && ty <= y1) m |= (uint64_t)xm << (7 * ty);
+        if (ty >= y0 && ty <= y1) m |= (uint64_t)xm << kRefineRowBase[ty];
     return m;
 }
 
@@ -387,8 +462,8 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
             }
         } else {
             /* every coarse candidate lies inside the window, so each is
-             * evaluated where it is; the mask bit of an out-of-limit one
-             * replaces its cost by 65535 (lanes 57-63 of slot 1 are off) */
+             * evaluated where it is; the mask bit of an out-of-limit one (or
+             * of a spare lane) replaces its cost by 65535 */
             static_assert(kVariant == kMeSurface || kPitch == kTwoStagePitch, "lane tables assume the two-stage pitch");
             uint64_t m[2];
             coarse_masks(lx0, lx1, ly0, ly1, m[0], m[1]);
@@ -423,7 +498,8 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
         } else {
             /* lane (tx, ty) starts at the byte address of (sx - 3, sy - 3)
              * plus 60 ty + tx; out-of-limit candidates (at most 17 px out)
-             * read the window's spare rows and are masked off */
+             * read the window's spare rows and are masked off, as are the
+             * spare lanes */
             const uint64_t m2 = refine_mask(sx, sy, lx0, lx1, ly0, ly1);
             const uint32_t e2 = c_refine.e[lane];
             const int a = 4 * kPitch * (sy + 12) + (sx + 13) + (int)(e2 & 0xffffu);
