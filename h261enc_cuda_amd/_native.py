@@ -60,6 +60,7 @@ OPT_XCD_REMAP = 4
 OPT_ASYNC_INPUT = 5
 OPT_DEVICE_BITSTREAM = 6
 OPT_OVERLAP_WRITER = 7
+OPT_CHAINS = 8
 
 
 class StreamParams(ctypes.Structure):

@@ -173,11 +173,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
     __shared__ WaveLds lds_all[kWavesPerGroup];
     __shared__ uint32_t tc[kTc]; /* run/level table: code | len << 16 */
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int n_waves = (p.n_mb + kMbsPerWave - 1) / kMbsPerWave;
+    const int n_waves = (p.n_mb - p.mb_first + kMbsPerWave - 1) / kMbsPerWave;
     const int n_groups = (n_waves + kWavesPerGroup - 1) / kWavesPerGroup;
     const int grp = launch_index(blockIdx.x, n_groups, p.xcd_remap);
     if (grp >= n_groups) return; /* the whole workgroup */
-    const int g0 = (grp * kWavesPerGroup + wid) * kMbsPerWave;
+    const int g0 = p.mb_first + (grp * kWavesPerGroup + wid) * kMbsPerWave;
     WaveLds &w = lds_all[wid];
 
     /* every load is issued before its value is needed: two rounds of
@@ -400,12 +400,12 @@ __global__ __launch_bounds__(256) void fetch_copy_kernel(const int32_t *ids, con
 
 int launch_bitstream(const BitLaunch &p, void *stream, void *ev_start, void *ev_stop)
 {
-    if (p.n_mb <= 0) {
+    if (p.n_mb - p.mb_first <= 0) {
         if (ev_start) (void)hipEventRecord((hipEvent_t)ev_start, (hipStream_t)stream);
         if (ev_stop) (void)hipEventRecord((hipEvent_t)ev_stop, (hipStream_t)stream);
         return 0;
     }
-    const int n_waves = (p.n_mb + kMbsPerWave - 1) / kMbsPerWave;
+    const int n_waves = (p.n_mb - p.mb_first + kMbsPerWave - 1) / kMbsPerWave;
     const int n_groups = (n_waves + kWavesPerGroup - 1) / kWavesPerGroup;
     const dim3 grid(launch_grid(n_groups, p.xcd_remap));
     if (ev_start || ev_stop)

@@ -163,6 +163,11 @@ struct h261g_engine {
     hipEvent_t wr_fork = nullptr, wr_join = nullptr;
     int overlap_writer = 2; /* 0 off, 1 on, 2 when a launch fills the device */
     bool writer_pending = false;
+    /* two chains: the second half of the MB space is stepped on chain_stream,
+     * forked from and joined back into the engine stream around a run */
+    hipStream_t chain_stream = nullptr;
+    hipEvent_t chain_fork = nullptr, chain_join = nullptr;
+    int chains = 0; /* 0 by size, 1 one chain, 2 two chains */
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_variant = kMeTwoStage;
@@ -432,21 +437,37 @@ int flush_writer(h261g_engine *e)
     return launch_bitstream(writer_launch(e), e->stream) ? H261G_ERR_DEVICE : H261G_OK;
 }
 
-int enqueue_step(h261g_engine *e, bool timed)
+/* The MB split of a two-chain step: the end of the open stream whose range
+ * ends nearest the middle of the MB space (ranges are contiguous, so both
+ * halves hold whole streams), or 0 for one chain. */
+int chain_split(const h261g_engine *e, bool timed)
 {
-    if (timed && e->ev_recorded - e->ev_read == h261g_engine::kEvRing) {
-        int rcode = read_oldest_times(e);
-        if (rcode) return rcode;
+    const int n = e->mb_used;
+    if (timed || e->chains == 1 || (e->chains == 0 && n < 2 * kFillWaves)) return 0;
+    int best = 0;
+    int64_t best_d = INT64_MAX;
+    for (const auto &s : e->slots) {
+        if (!s.open) continue;
+        const int end = s.mb_base + s.n_mb;
+        const int64_t d = end > n / 2 ? end - n / 2 : n / 2 - end;
+        if (end < n && d < best_d) {
+            best = end;
+            best_d = d;
+        }
     }
-    hipEvent_t *ev = e->ev[e->ev_recorded % h261g_engine::kEvRing];
+    return best;
+}
+
+/* one frame step of the streams whose MBs lie in [lo, hi), on stream st */
+int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, bool timed, hipEvent_t *ev)
+{
     const int ns = stream_count(e);
-    const int n_mb = e->mb_used;
-    hipStream_t st = e->stream;
     const int fused = e->fused_tails;
     MeLaunch ml{};
     ml.streams = e->desc;
     ml.mb_stream = e->mb_stream;
-    ml.n_mb = n_mb;
+    ml.n_mb = hi;
+    ml.mb_first = lo;
     ml.result = e->me;
     ml.mode = e->mode;
     ml.rc = e->rc;
@@ -454,9 +475,10 @@ int enqueue_step(h261g_engine *e, bool timed)
     ml.variant = e->me_variant;
     ml.fuse_tail = fused;
     ml.xcd_remap = e->xcd_remap & 1;
-    /* the previous step's bitstreams beside this step's motion estimation
-     * (with nothing pending every MB is skipped) */
-    const bool overlap = e->device_bits && (e->overlap_writer == 1 || (e->overlap_writer == 2 && n_mb >= kFillWaves));
+    /* one chain: the previous step's bitstreams beside this step's motion
+     * estimation (with nothing pending every MB is skipped) */
+    const bool overlap =
+        whole && e->device_bits && (e->overlap_writer == 1 || (e->overlap_writer == 2 && hi - lo >= kFillWaves));
     if (overlap) {
         if (!ok(hipEventRecord(e->wr_fork, st), "event") || !ok(hipStreamWaitEvent(e->wr_stream, e->wr_fork, 0), "wait") ||
             launch_bitstream(writer_launch(e), e->wr_stream, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr) ||
@@ -469,7 +491,8 @@ int enqueue_step(h261g_engine *e, bool timed)
     MbLaunch bl{};
     bl.streams = e->desc;
     bl.mb_stream = e->mb_stream;
-    bl.n_mb = n_mb;
+    bl.n_mb = hi;
+    bl.mb_first = lo;
     bl.mode = e->mode;
     bl.denoise = e->dn;
     bl.rc = e->rc;
@@ -478,6 +501,7 @@ int enqueue_step(h261g_engine *e, bool timed)
     /* 0 = by batch size: one MB per wave while the launch is too small to
      * fill the device, up to four when there are MBs to spare (fewer waves,
      * fewer statistic flushes) */
+    const int n_mb = hi - lo;
     bl.mbs_per_wave = e->mbs_per_wave ? e->mbs_per_wave : (n_mb >= 4 * kFillWaves ? 4 : (n_mb >= 2 * kFillWaves ? 2 : 1));
     bl.xcd_remap = (e->xcd_remap >> 1) & 1;
     if (launch_mb(bl, st, timed ? ev[2] : nullptr, timed ? ev[3] : nullptr)) return H261G_ERR_DEVICE;
@@ -486,11 +510,12 @@ int enqueue_step(h261g_engine *e, bool timed)
     MbHdr *hdr = e->device_bits ? e->hdr : nullptr;
     if (fused) {
         if (launch_frame_tail(e->desc, e->seq, e->dn, e->rc, e->side, hdr, ns, st, timed ? ev[4] : nullptr,
-                              timed ? ev[5] : nullptr))
+                              timed ? ev[5] : nullptr, whole ? 0 : lo, whole ? INT32_MAX : hi))
             return H261G_ERR_DEVICE;
     } else {
-        /* separate kernels: "other" spans the three end-of-frame launches
-         * (rate-control begin, between the two big kernels, is not timed) */
+        /* separate kernels (one chain only): "other" spans the three
+         * end-of-frame launches (rate-control begin, between the two big
+         * kernels, is not timed) */
         if (timed) (void)hipEventRecord(ev[4], st);
         if (launch_denoise_update(e->dn, e->desc, ns, st)) return H261G_ERR_DEVICE;
         if (launch_frame_bits(e->rc, e->desc, ns, e->side, nullptr, st, e->seq, hdr)) return H261G_ERR_DEVICE;
@@ -498,14 +523,47 @@ int enqueue_step(h261g_engine *e, bool timed)
         if (timed) (void)hipEventRecord(ev[5], st);
     }
     if (hdr && !overlap) {
-        if (launch_bitstream(writer_launch(e), st, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr))
-            return H261G_ERR_DEVICE;
+        BitLaunch wl = writer_launch(e);
+        wl.mb_first = lo;
+        wl.n_mb = hi;
+        if (launch_bitstream(wl, st, timed ? ev[6] : nullptr, timed ? ev[7] : nullptr)) return H261G_ERR_DEVICE;
     } else if (!hdr && timed) {
         (void)hipEventRecord(ev[6], st);
         (void)hipEventRecord(ev[7], st);
     }
-    e->writer_pending = overlap;
-    if (timed) e->ev_recorded++;
+    if (whole) e->writer_pending = overlap;
+    return H261G_OK;
+}
+
+/* k frame steps on the engine stream: one chain, or two chains forked onto
+ * chain_stream and joined back after the k steps (the chains share no data,
+ * so they need not meet between steps) */
+int enqueue_steps(h261g_engine *e, int k, bool timed)
+{
+    const int split = e->fused_tails ? chain_split(e, timed) : 0;
+    hipStream_t st = e->stream;
+    if (!split) {
+        for (int i = 0; i < k; i++) {
+            if (timed && e->ev_recorded - e->ev_read == h261g_engine::kEvRing) {
+                int rcode = read_oldest_times(e);
+                if (rcode) return rcode;
+            }
+            hipEvent_t *ev = e->ev[e->ev_recorded % h261g_engine::kEvRing];
+            int rcode = enqueue_chain(e, st, 0, e->mb_used, true, timed, ev);
+            if (rcode) return rcode;
+            if (timed) e->ev_recorded++;
+        }
+        return H261G_OK;
+    }
+    if (!ok(hipEventRecord(e->chain_fork, st), "event") || !ok(hipStreamWaitEvent(e->chain_stream, e->chain_fork, 0), "wait"))
+        return H261G_ERR_DEVICE;
+    for (int i = 0; i < k; i++) {
+        int rcode = enqueue_chain(e, st, 0, split, false, false, nullptr);
+        if (!rcode) rcode = enqueue_chain(e, e->chain_stream, split, e->mb_used, false, false, nullptr);
+        if (rcode) return rcode;
+    }
+    if (!ok(hipEventRecord(e->chain_join, e->chain_stream), "event") || !ok(hipStreamWaitEvent(st, e->chain_join, 0), "wait"))
+        return H261G_ERR_DEVICE;
     return H261G_OK;
 }
 
@@ -576,6 +634,10 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         if (value < 0 || value > 2) return H261G_ERR_ARG;
         e->overlap_writer = value;
         break;
+    case H261G_OPT_CHAINS:
+        if (value < 0 || value > 2) return H261G_ERR_ARG;
+        e->chains = value;
+        break;
     case H261G_OPT_DEVICE_BITSTREAM:
         if (value < 0 || value > 2) return H261G_ERR_ARG;
         /* bitstream buffers are sized at open */
@@ -629,6 +691,9 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipStreamCreateWithFlags(&e->wr_stream, hipStreamNonBlocking), "stream") &&
                 ok(hipEventCreateWithFlags(&e->wr_fork, hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->wr_join, hipEventDisableTiming), "event") &&
+                ok(hipStreamCreateWithFlags(&e->chain_stream, hipStreamNonBlocking), "stream") &&
+                ok(hipEventCreateWithFlags(&e->chain_fork, hipEventDisableTiming), "event") &&
+                ok(hipEventCreateWithFlags(&e->chain_join, hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->compute_ev, hipEventDisableTiming), "event") &&
                 ok(hipMalloc((void **)&e->desc, sizeof(StreamDesc) * max_streams), "hipMalloc") &&
                 ok(hipMalloc((void **)&e->seq, sizeof(SeqState) * max_streams), "hipMalloc") &&
@@ -657,6 +722,7 @@ int h261g_engine_destroy(h261g_engine_t *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->in_stream) (void)hipStreamSynchronize(e->in_stream);
     if (e->wr_stream) (void)hipStreamSynchronize(e->wr_stream);
+    if (e->chain_stream) (void)hipStreamSynchronize(e->chain_stream);
     for (auto &s : e->slots)
         if (s.gen_ev) (void)hipEventDestroy(s.gen_ev);
     for (auto &c : e->chunks) hipFree(c.base);
@@ -692,6 +758,9 @@ int h261g_engine_destroy(h261g_engine_t *e)
     if (e->wr_stream) (void)hipStreamDestroy(e->wr_stream);
     if (e->wr_fork) (void)hipEventDestroy(e->wr_fork);
     if (e->wr_join) (void)hipEventDestroy(e->wr_join);
+    if (e->chain_stream) (void)hipStreamDestroy(e->chain_stream);
+    if (e->chain_fork) (void)hipEventDestroy(e->chain_fork);
+    if (e->chain_join) (void)hipEventDestroy(e->chain_join);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return H261G_OK;
@@ -1018,7 +1087,10 @@ int h261g_engine_step(h261g_engine_t *e)
     int n = 0;
     for (auto &s : e->slots)
         if (s.open && (s.pending || s.seq)) n++;
-    int rcode = enqueue_step(e, e->profile);
+    /* a two-chain step runs its writers in-chain: a writer left pending by
+     * one-chain steps goes first */
+    if (chain_split(e, e->profile) && flush_writer(e)) return H261G_ERR_DEVICE;
+    int rcode = enqueue_steps(e, 1, e->profile);
     if (rcode) return rcode;
     for (auto &s : e->slots) s.fresh = false;
 
@@ -1040,6 +1112,7 @@ int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
         }
         return H261G_OK;
     }
+    if (chain_split(e, false) && flush_writer(e)) return H261G_ERR_DEVICE;
     if (!e->graph_valid) {
         if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
         if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -1052,8 +1125,7 @@ int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
             hipGraphExec_t &x = k == 1 ? e->graph_exec : e->graph_exec_k;
             if (!ok(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "capture"))
                 return H261G_ERR_DEVICE;
-            int rcode = 0;
-            for (int i = 0; i < k && !rcode; i++) rcode = enqueue_step(e, false);
+            int rcode = enqueue_steps(e, k, false);
             hipError_t ce = hipStreamEndCapture(e->stream, &g);
             if (rcode || !ok(ce, "end capture")) return H261G_ERR_DEVICE;
             if (!ok(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "instantiate")) return H261G_ERR_DEVICE;

@@ -62,11 +62,11 @@ __global__ __launch_bounds__(64) void denoise_update_kernel(DenoiseState *dn_all
 
 __global__ __launch_bounds__(64 * fc::kTailWaves) void frame_tail_kernel(StreamDesc *streams, SeqState *seq, DenoiseState *dn_all,
                                                          RcState *rc_all, const MbSide *side, MbHdr *hdr,
-                                                         int32_t *bytes_out)
+                                                         int32_t *bytes_out, int32_t mb_lo, int32_t mb_hi)
 {
     const int s = blockIdx.x;
     StreamDesc &d = streams[s];
-    if (!(d.flags & kStreamActive)) return;
+    if (!(d.flags & kStreamActive) || d.mb_base < mb_lo || d.mb_base >= mb_hi) return;
     fc::frame_tail_group(d, seq[s], dn_all + s, rc_all[s], side, hdr, bytes_out ? bytes_out + s : nullptr,
                          threadIdx.x >> 6, threadIdx.x & 63);
 }
@@ -83,7 +83,8 @@ __global__ void advance_kernel(StreamDesc *streams, SeqState *seq, int n)
 }  // namespace
 
 int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
-                      MbHdr *hdr, int n_streams, void *stream, void *ev_start, void *ev_stop)
+                      MbHdr *hdr, int n_streams, void *stream, void *ev_start, void *ev_stop, int32_t mb_lo,
+                      int32_t mb_hi)
 {
     if (n_streams <= 0) {
         if (ev_start) { /* nothing to launch: an empty timed interval */
@@ -95,10 +96,11 @@ int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcSt
     int32_t *no_bytes = nullptr;
     if (ev_start)
         hipExtLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64 * fc::kTailWaves), 0, (hipStream_t)stream,
-                              (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0, streams, seq, dn, rc, side, hdr, no_bytes);
+                              (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0, streams, seq, dn, rc, side, hdr, no_bytes, mb_lo,
+                              mb_hi);
     else
         hipLaunchKernelGGL(frame_tail_kernel, dim3(n_streams), dim3(64 * fc::kTailWaves), 0, (hipStream_t)stream, streams, seq, dn, rc,
-                           side, hdr, no_bytes);
+                           side, hdr, no_bytes, mb_lo, mb_hi);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

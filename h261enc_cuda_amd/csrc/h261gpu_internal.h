@@ -157,11 +157,13 @@ H261G_HD inline int launch_index(int i, int n, bool remap) { return remap ? xcd_
 
 /* one wave per MB: two-stage SAD search, the three SATD costs, and (when
  * mode != nullptr) the reference mode decision for the MB.
- * Launch covers n_mb global MBs; mb_stream maps a global MB to its stream. */
+ * Launch covers global MBs [mb_first, n_mb); mb_stream maps a global MB to
+ * its stream.  Per-MB arrays are indexed by the global MB. */
 struct MeLaunch {
     const StreamDesc *streams;
     const int32_t *mb_stream;
-    int32_t n_mb;
+    int32_t n_mb;       /* end of the launch's MB range */
+    int32_t mb_first;   /* start of the launch's MB range */
     MeResult *result;   /* [n_mb] */
     int8_t *mv_start;   /* optional [2*n_mb]: refined vector (me_gpu_t starting vector) */
     MbMode *mode;       /* optional [n_mb]: device mode decision */
@@ -184,7 +186,8 @@ int launch_me(const MeLaunch &p, void *stream, void *ev_start = nullptr, void *e
 struct MbLaunch {
     const StreamDesc *streams;
     const int32_t *mb_stream;
-    int32_t n_mb;
+    int32_t n_mb;                 /* launch covers global MBs [mb_first, n_mb) */
+    int32_t mb_first;
     const MbMode *mode;           /* [n_mb] */
     DenoiseState *denoise;        /* per stream */
     RcState *rc;                  /* per stream: kStreamDeviceQuant, fused tail */
@@ -211,15 +214,18 @@ int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, con
 int launch_advance(StreamDesc *streams, SeqState *seq, int n_streams, void *stream);
 /* per stream, one wave: denoise update, frame size, rate-control update and
  * advance in one launch; with hdr, also each coded MB's bitstream position
- * and header codes, and the picture and GOB headers in SeqState.bits */
+ * and header codes, and the picture and GOB headers in SeqState.bits.
+ * Streams whose MB range starts outside [mb_lo, mb_hi) are left alone. */
 int launch_frame_tail(StreamDesc *streams, SeqState *seq, DenoiseState *dn, RcState *rc, const MbSide *side,
-                      MbHdr *hdr, int n_streams, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
+                      MbHdr *hdr, int n_streams, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr,
+                      int32_t mb_lo = 0, int32_t mb_hi = 0x7fffffff);
 
 /* one wave per MB: the coded MBs' bits (headers from the frame tail, then
  * the TCOEFF codes of their levels) into their streams' bitstreams */
 struct BitLaunch {
     const int32_t *mb_stream;
-    int32_t n_mb;
+    int32_t n_mb;     /* launch covers global MBs [mb_first, n_mb) */
+    int32_t mb_first;
     const SeqState *seq;
     MbHdr *hdr;
     const MbSide *side;

@@ -264,11 +264,11 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     const float post_c = (float)((double)c_fwd_post[rc] * 0.125);
     const float pre_c = c_inv_pre[rc];
 
-    const int n_wg = (p.n_mb + p.mbs_per_wave - 1) / p.mbs_per_wave;
+    const int n_wg = (p.n_mb - p.mb_first + p.mbs_per_wave - 1) / p.mbs_per_wave;
     const int wg = launch_index(blockIdx.x, n_wg, p.xcd_remap);
     if (wg >= n_wg) return;
     for (int grp = 0; grp * G < p.mbs_per_wave; grp++) {
-        const int g0 = wg * p.mbs_per_wave + grp * G;
+        const int g0 = p.mb_first + wg * p.mbs_per_wave + grp * G;
         if (g0 >= p.n_mb) break;
         const int s0 = first_stream(p, g0, G);
         if (s0 < 0) continue;
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 
 int launch_mb(const MbLaunch &p, void *stream, void *ev_start, void *ev_stop)
 {
-    if (p.n_mb <= 0) {
+    if (p.n_mb - p.mb_first <= 0) {
         if (ev_start) { /* nothing to launch: an empty timed interval */
             (void)hipEventRecord((hipEvent_t)ev_start, (hipStream_t)stream);
             (void)hipEventRecord((hipEvent_t)ev_stop, (hipStream_t)stream);
@@ -637,7 +637,7 @@ int launch_mb(const MbLaunch &p, void *stream, void *ev_start, void *ev_stop)
     const int per = p.mbs_per_wave > 0 ? p.mbs_per_wave : 1;
     MbLaunch q = p;
     q.mbs_per_wave = per;
-    const int grid = launch_grid((p.n_mb + per - 1) / per, p.xcd_remap);
+    const int grid = launch_grid((p.n_mb - p.mb_first + per - 1) / per, p.xcd_remap);
     /* groups of 4 MBs fill all 64 lanes; otherwise one MB per pass */
     auto k = per % 4 == 0 ? mb_kernel<4> : mb_kernel<1>;
     if (ev_start)

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
     __shared__ __align__(16) uint32_t curw[kVariant == kMeSurface ? 64 : 4];
     __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
 
-    const int g = launch_index(blockIdx.x, p.n_mb, p.xcd_remap);
+    const int g = p.mb_first + launch_index(blockIdx.x, p.n_mb - p.mb_first, p.xcd_remap);
     if (g >= p.n_mb) return;
     const int lane = threadIdx.x;
     const int s = p.mb_stream[g];
@@ -614,14 +614,14 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
 
 int launch_me(const MeLaunch &p, void *stream, void *ev_start, void *ev_stop)
 {
-    if (p.n_mb <= 0) {
+    if (p.n_mb - p.mb_first <= 0) {
         if (ev_start) { /* nothing to launch: an empty timed interval */
             (void)hipEventRecord((hipEvent_t)ev_start, (hipStream_t)stream);
             (void)hipEventRecord((hipEvent_t)ev_stop, (hipStream_t)stream);
         }
         return 0;
     }
-    const dim3 grid(launch_grid(p.n_mb, p.xcd_remap));
+    const dim3 grid(launch_grid(p.n_mb - p.mb_first, p.xcd_remap));
     auto k = p.variant == kMeSurface ? me_kernel<kMeSurface> : me_kernel<kMeTwoStage>;
     if (ev_start)
         hipExtLaunchKernelGGL(k, grid, dim3(64), 0, (hipStream_t)stream, (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0,

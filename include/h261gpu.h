@@ -143,6 +143,14 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                       stream changes launch a pending writer first); 0: at the end of its
                                       own step; 2 (default): 1 when a step has enough macroblocks to
                                       fill the device, else 0 (a short step is latency bound) */
+#define H261G_OPT_CHAINS 8 /* 2: the streams are split at a stream boundary near the middle of the
+                              macroblock space into two halves, each stepped by its own sequence
+                              of launches on its own HIP stream (the halves share nothing, so one
+                              half's kernel tails overlap the other half's work; the bitstream
+                              writer then runs at the end of each half's step); 1: one sequence;
+                              0 (default): 2 when a step has at least twice the macroblocks that
+                              fill the device, else 1.  Profiled steps
+                              (h261g_engine_profile) always run as one sequence. */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */

@@ -231,6 +231,63 @@ def test_engine_graph_run_matches_stepwise():
     b.destroy()
 
 
+def test_engine_two_chains_match_one_chain(oracle):
+    """Two chains (the MB space split at a stream boundary, each half stepped
+    on its own HIP stream) code every stream as one chain does, stepwise and
+    under graph replay, across a close and a re-packing; one stream is also
+    checked against the oracle."""
+    from h261enc_cuda_amd._native import OPT_CHAINS
+    specs = [(CIF, 301, 0, 256000), (QCIF, 302, 0, 64000), (QCIF, 303, 6, 0), (CIF, 304, 0, 384000),
+             (QCIF, 305, 0, 128000)]
+    n = 10
+    engines = []
+    for chains in (1, 2):
+        eng = Engine(max_streams=8)
+        eng.config(OPT_CHAINS, chains)
+        sids = []
+        for size, seed, fixed, rate in specs:
+            sid = eng.open(*size, fixed_quant=fixed, bitrate=rate or 512000, vbv_size=rate or 512000)
+            eng.generate_input(sid, seed, 3 * n, sigma=2, motion=15)
+            sids.append(sid)
+        engines.append((eng, sids))
+    announce("engine two chains vs one chain")
+
+    def compare(what):
+        (a, ia), (b, ib) = engines
+        for k, (sa, sb) in enumerate(zip(ia, ib)):
+            if sa is None:
+                continue
+            assert a.bitstream(sa) == b.bitstream(sb), f"{what}: stream {k} bitstream"
+            assert a.info(sa).quant == b.info(sb).quant, f"{what}: stream {k} quantiser"
+            ra, rb = a.recon(sa), b.recon(sb)
+            for p in range(3):
+                assert np.array_equal(ra[p], rb[p]), f"{what}: stream {k} recon"
+
+    size, seed, fixed, rate = specs[3]
+    os_ = oracle.stream(*size, fixed_quant=fixed, bitrate=rate, vbv=rate)
+    for t in range(n):
+        for eng, _ in engines:
+            eng.step()
+            eng.sync()
+        compare(f"step {t}")
+        ob, _, _, _, _ = os_.encode(*content_frame(seed, *size, t, sigma=2, motion=15))
+        assert engines[1][0].bitstream(engines[1][1][3]) == ob, f"two chains vs oracle, frame {t}"
+    for eng, _ in engines:
+        eng.run(5)
+        eng.sync()
+    compare("graph replay")
+    for eng, sids in engines:
+        eng.close(sids[1])
+        sids[1] = None
+        eng.compact()
+        eng.run(4)
+        eng.step()
+        eng.sync()
+    compare("after close and re-packing")
+    for eng, _ in engines:
+        eng.destroy()
+
+
 def test_engine_launch_shapes_agree(oracle):
     """Fused and unfused frame tails, 1/3/8 MBs per wave (workgroups that
     span several streams) and both search kernels code identical bytes."""
