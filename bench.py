@@ -46,19 +46,19 @@ ME_BYTES = {396: 352 * 288 * 2 + 44 * 396, 99: 176 * 144 * 2 + 44 * 99}
 MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 384 + 8)}
 
 
-# HBM bytes per macroblock actually moved by each kernel, with and without
-# its XCD remap: FETCH_SIZE and WRITE_SIZE per dispatch from two rocprofv3
-# --pmc passes over `bench.py --steps 8 --no-cpu --no-plain --no-extra
-# --copy-probe --shape 0,R` (R = 3 and 2; all 512 streams active, 126,720 MBs
-# per dispatch), each scaled by what the same counter reads for the 256 MiB
-# device copy of --copy-probe, divided by the dispatch's MB count
-# (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation:
-# 98.3 MB read + 9.7 MB written remapped, 509.3 + 16.3 MB not; the coder
-# (remapped) 105.2 + 135.1 MB; the bitstream writer (not remapped)
-# 21.6 + 6.7 MB.
-TRAFFIC_PER_MB = {("me_kernel", True): (98.31e6 + 9.70e6) / 126720, ("me_kernel", False): (509.25e6 + 16.29e6) / 126720,
-                  ("mb_kernel", True): (105.20e6 + 135.09e6) / 126720,
-                  ("bitstream_kernel", False): (21.55e6 + 6.68e6) / 126720}
+# HBM bytes per macroblock actually moved by each kernel with the XCD remap
+# of motion estimation and coder (the autotuner's choice): FETCH_SIZE and
+# WRITE_SIZE per dispatch from two rocprofv3 --pmc passes over `bench.py
+# --steps 8 --no-cpu --no-plain --no-extra --copy-probe --shape 0,3` (the
+# event pass's last 8 steps: all 512 streams active, 126,720 MBs per
+# dispatch), each scaled by what the same counter reads for the 256 MiB device
+# copy of --copy-probe, divided by the dispatch's MB count
+# (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation
+# 95.6 MB read + 9.7 MB written, the coder 108.0 + 137.4 MB, the bitstream
+# writer (not remapped) 20.1 + 6.5 MB.  Other remap settings: not measured.
+TRAFFIC_PER_MB = {("me_kernel", True): (95.57e6 + 9.70e6) / 126720,
+                  ("mb_kernel", True): (108.03e6 + 137.40e6) / 126720,
+                  ("bitstream_kernel", False): (20.07e6 + 6.47e6) / 126720}
 REMAP_BIT = {"me_kernel": 1, "mb_kernel": 2, "bitstream_kernel": 4}
 
 
