@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the hard-motion and serving lines")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--compact-every", type=int, default=30,
+                    help="steps between closing finished streams and re-packing the MB space")
     ap.add_argument("--shape", default=None,
                     help="MBS_PER_WAVE,XCD_REMAP: a fixed launch shape instead of the autotuner's (profiling)")
     args = ap.parse_args()
@@ -321,7 +323,7 @@ def main():
     if W:
         eng.run(1)
     eng.sync()
-    dt, frames, mbs = run_job(eng, ids, specs, K, start=W)
+    dt, frames, mbs = run_job(eng, ids, specs, K, start=W, compact_every=args.compact_every)
     eng.destroy()
     cif_equiv = mbs / 396.0
     value = cif_equiv / dt
@@ -346,7 +348,7 @@ def main():
         if not first:
             first.update(eng.kernel_times(), done=done)
 
-    run_job(eng, ids, specs, K, start=W, after_chunk=after_chunk)
+    run_job(eng, ids, specs, K, start=W, compact_every=args.compact_every, after_chunk=after_chunk)
     prof = eng.kernel_times()
     eng.destroy()
 
@@ -362,7 +364,7 @@ def main():
     if W:
         eng.run(1)
     eng.sync()
-    dt_l, _, mbs_l = run_job(eng, ids, specs, K, start=W)
+    dt_l, _, mbs_l = run_job(eng, ids, specs, K, start=W, compact_every=args.compact_every)
     eng.destroy()
     levels_fps = mbs_l / 396.0 / dt_l
 

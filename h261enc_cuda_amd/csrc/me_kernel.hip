@@ -416,7 +416,12 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
     /* buffer loads over the luma plane: a 32-bit lane offset, and in the
      * interior case a scalar row offset (addresses stay in the plane, so its
      * range check never applies) */
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)ref, 0, W * H, kBufferDword3);
+    /* the plane's address and size are wave-uniform: said so explicitly,
+     * else the resource is built per lane and each load loops over lanes */
+    const uint64_t ref_u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uintptr_t)ref >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uintptr_t)ref);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(uintptr_t)ref_u, 0, __builtin_amdgcn_readfirstlane(W * H), kBufferDword3);
     if (by >= 15 && by + 30 < H) {
         const uint32_t lo = (uint32_t)(rr * W + fx);
 #pragma unroll
