@@ -158,51 +158,94 @@ __device__ __forceinline__ int tcoeff_bits(const uint8_t *tlen, int run, int lev
     return (run < kTcoeffRuns && a < kTcoeffLevels) ? (int)tlen[run * kTcoeffLevels + a] : kEscapeBits;
 }
 
+/* One MB of a group as the lanes of its blocks need it: lane mi computes it
+ * once per group (the stream's descriptor, the mode, the picture quantiser)
+ * and keeps it in LDS, where every pass reads it back. */
+struct MbInfo {
+    const uint8_t *cur[3];
+    const uint8_t *ref[3];
+    uint8_t *rec[3];
+    const int32_t *offs;   /* the previous frame's denoise offsets of the MB's type */
+    int32_t s;             /* stream; -1: no active stream */
+    int32_t bx, by, width; /* luma position and width */
+    int32_t quant, intra, filter, mvx, mvy;
+    int32_t pad[3];
+};
+static_assert(sizeof(MbInfo) == 128, "four 32-byte rows");
+
+__device__ __forceinline__ MbInfo mb_info(const MbLaunch &p, int g)
+{
+    MbInfo o{};
+    o.s = g < p.n_mb ? p.mb_stream[g] : -1;
+    if (o.s < 0) return o;
+    const StreamDesc &d = p.streams[o.s];
+    if (!(d.flags & kStreamActive)) {
+        o.s = -1;
+        return o;
+    }
+    const MbMode md = p.mode[g];
+    const int m = g - d.mb_base;
+    o.bx = (m % d.mb_w) * 16;
+    o.by = (m / d.mb_w) * 16;
+    o.width = d.width;
+    const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[o.s].pic_quant : d.pic_quant;
+    o.quant = clampi(picq + md.quant_adjust, 1, 31); /* gpu_device.cu:625-627 */
+    o.intra = md.type == H261_MB_TYPE_INTRA;
+    o.filter = (md.flags & H261_MB_FILTER) != 0;
+    o.mvx = md.mvx;
+    o.mvy = md.mvy;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        o.cur[k] = d.cur[k];
+        o.ref[k] = d.ref[k];
+        o.rec[k] = d.rec[k];
+    }
+    o.offs = o.intra ? p.denoise[o.s].intra_offset : p.denoise[o.s].inter_offset;
+    return o;
+}
+
 /* Per-lane view of the block a lane works on in one pass: lane (k, r) of a
  * pass takes block 8 * pass + k of its group of G macroblocks (6 blocks
  * each), row or column r. */
 struct LaneBlock {
     bool act;      /* the block exists and its stream is active */
-    int g, s;      /* global MB, stream */
+    int s;         /* stream */
     int b;         /* block in the MB: 0-3 luma, 4 Cb, 5 Cr */
     int slot;      /* statistics slot: 0 = the group's first stream, 1 = another */
     bool intra, filter;
     int quant, t;  /* t: 0 intra, 1 inter (statistics table) */
     int px, py, stride, mvx, mvy, plane;
+    int mi;        /* MB of the group (its MbInfo holds the plane pointers) */
 };
 
-__device__ __forceinline__ LaneBlock lane_block(const MbLaunch &p, int g, int b, int s0)
+__device__ __forceinline__ LaneBlock lane_block(const MbInfo *info, int bi, int n_blocks, int s0)
 {
     LaneBlock L{};
-    L.g = g;
+    const int mi = bi / 6, b = bi - 6 * mi;
+    const MbInfo &o = info[bi < n_blocks ? mi : 0];
     L.b = b;
-    L.s = g < p.n_mb ? p.mb_stream[g] : -1;
+    L.mi = mi;
+    L.s = bi < n_blocks ? o.s : -1;
     L.act = L.s >= 0;
-    const StreamDesc &d = p.streams[L.act ? L.s : 0];
-    L.act = L.act && (d.flags & kStreamActive);
     if (!L.act) return L;
     L.slot = L.s == s0 ? 0 : 1;
-    const MbMode md = p.mode[g];
-    const int m = g - d.mb_base;
-    const int bx = (m % d.mb_w) * 16, by = (m / d.mb_w) * 16;
-    const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[L.s].pic_quant : d.pic_quant;
-    L.quant = clampi(picq + md.quant_adjust, 1, 31); /* gpu_device.cu:625-627 */
-    L.intra = md.type == H261_MB_TYPE_INTRA;
-    L.filter = (md.flags & H261_MB_FILTER) != 0;
+    L.quant = o.quant;
+    L.intra = o.intra;
+    L.filter = o.filter;
     L.t = L.intra ? 0 : 1;
     L.plane = b < 4 ? 0 : b - 3;
     if (b < 4) {
-        L.px = bx + (b & 1) * 8;
-        L.py = by + ((b >> 1) & 1) * 8;
-        L.stride = d.width;
-        L.mvx = md.mvx;
-        L.mvy = md.mvy;
+        L.px = o.bx + (b & 1) * 8;
+        L.py = o.by + ((b >> 1) & 1) * 8;
+        L.stride = o.width;
+        L.mvx = o.mvx;
+        L.mvy = o.mvy;
     } else { /* chroma vector = luma vector / 2, truncated (mbgpu:266-269) */
-        L.px = bx >> 1;
-        L.py = by >> 1;
-        L.stride = d.width >> 1;
-        L.mvx = md.mvx / 2;
-        L.mvy = md.mvy / 2;
+        L.px = o.bx >> 1;
+        L.py = o.by >> 1;
+        L.stride = o.width >> 1;
+        L.mvx = o.mvx / 2;
+        L.mvy = o.mvy / 2;
     }
     return L;
 }
@@ -250,6 +293,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     __shared__ int32_t stat[2][2][64]; /* [stream slot][intra, inter][coefficient] */
     __shared__ int32_t scount[2][2];
     __shared__ __align__(16) uint32_t tlen32[kTlenWords];
+    __shared__ MbInfo info[G];
     const uint8_t *tlen = (const uint8_t *)tlen32;
 
     const int lane = threadIdx.x;
@@ -274,13 +318,13 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
         if (s0 < 0) continue;
         int s1 = -1; /* the group's other stream, if any */
         uint32_t coded_bits = 0;
+        if (lane < G) info[lane] = mb_info(p, g0 + lane);
         __syncthreads();
 
 #pragma unroll 1
         for (int pass = 0; pass < kPasses; pass++) {
             const int bi = pass * 8 + k8; /* block of the group */
-            const int mi = bi / 6;
-            const LaneBlock L = lane_block(p, bi < kBlocks ? g0 + mi : p.n_mb, bi - 6 * mi, s0);
+            const LaneBlock L = lane_block(info, bi, kBlocks, s0);
             if (L.act && L.slot) s1 = L.s;
 
             /* ---- phase 1: loads first (current row, reference rows rc-1..rc+1
@@ -291,17 +335,17 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             uint32_t offp[4]; /* the offsets as UInt16 (mbgpu:525-528), two per register */
             uint32_t predw[2] = {0, 0};
             if (L.act) {
-                const StreamDesc &d = p.streams[L.s];
-                const int32_t *offs = L.t == 0 ? p.denoise[L.s].intra_offset : p.denoise[L.s].inter_offset;
+                const MbInfo &mbi = info[L.mi];
+                const int32_t *offs = mbi.offs;
                 uint32_t offw[8];
 #pragma unroll
                 for (int r = 0; r < 8; r++) offw[r] = (uint32_t)offs[rc + 8 * r];
-                const uint2 cw = *(const uint2 *)(d.cur[L.plane] + (size_t)(L.py + rc) * L.stride + L.px);
+                const uint2 cw = *(const uint2 *)(mbi.cur[L.plane] + (size_t)(L.py + rc) * L.stride + L.px);
                 const int rx = L.px + L.mvx;
                 const int a = rx & ~3, o = rx & 3;
                 /* the third dword is needed only when o > 0, and then it lies in the row */
                 const int a2 = a + 8 <= L.stride - 4 ? a + 8 : a;
-                const uint8_t *rbase = d.ref[L.plane] + (size_t)(L.py + L.mvy) * L.stride;
+                const uint8_t *rbase = mbi.ref[L.plane] + (size_t)(L.py + L.mvy) * L.stride;
                 uint32_t rw[3][3];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -432,8 +476,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 
             /* lane (block, row rc) holds row rc of the block's prediction */
             if (L.act) {
-                const StreamDesc &d = p.streams[L.s];
-                uint8_t *dst = d.rec[L.plane] + (size_t)L.py * L.stride + L.px;
+                uint8_t *dst = info[L.mi].rec[L.plane] + (size_t)L.py * L.stride + L.px;
                 if (!coded) {
                     *(uint2 *)(dst + (size_t)rc * L.stride) = make_uint2(predw[0], predw[1]);
                 } else {
