@@ -169,9 +169,24 @@ struct MbInfo {
     int32_t s;             /* stream; -1: no active stream */
     int32_t bx, by, width; /* luma position and width */
     int32_t quant, intra, filter, mvx, mvy;
-    int32_t pad[3];
+    int32_t type, flags; /* the mode's MB type and flags */
+    int32_t pad;
 };
 static_assert(sizeof(MbInfo) == 128, "four 32-byte rows");
+
+/* MB positions (x | y << 8, in MBs) of the two picture formats H.261 has:
+ * CIF (22 x 18 MBs) from entry 0, QCIF (11 x 9) from entry 396 */
+struct MbPos {
+    uint16_t v[396 + 99];
+};
+constexpr MbPos make_mbpos()
+{
+    MbPos t{};
+    for (int m = 0; m < 396; m++) t.v[m] = (uint16_t)(m % 22 | (m / 22) << 8);
+    for (int m = 0; m < 99; m++) t.v[396 + m] = (uint16_t)(m % 11 | (m / 11) << 8);
+    return t;
+}
+__constant__ MbPos c_mbpos = make_mbpos();
 
 __device__ __forceinline__ MbInfo mb_info(const MbLaunch &p, int g)
 {
@@ -185,8 +200,14 @@ __device__ __forceinline__ MbInfo mb_info(const MbLaunch &p, int g)
     }
     const MbMode md = p.mode[g];
     const int m = g - d.mb_base;
-    o.bx = (m % d.mb_w) * 16;
-    o.by = (m / d.mb_w) * 16;
+    if ((d.mb_w == 22 && d.mb_h == 18) || (d.mb_w == 11 && d.mb_h == 9)) {
+        const int pos = c_mbpos.v[(d.mb_w == 22 ? 0 : 396) + m];
+        o.bx = (pos & 0xff) * 16;
+        o.by = (pos >> 8) * 16;
+    } else {
+        o.bx = (m % d.mb_w) * 16;
+        o.by = (m / d.mb_w) * 16;
+    }
     o.width = d.width;
     const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[o.s].pic_quant : d.pic_quant;
     o.quant = clampi(picq + md.quant_adjust, 1, 31); /* gpu_device.cu:625-627 */
@@ -194,6 +215,8 @@ __device__ __forceinline__ MbInfo mb_info(const MbLaunch &p, int g)
     o.filter = (md.flags & H261_MB_FILTER) != 0;
     o.mvx = md.mvx;
     o.mvy = md.mvy;
+    o.type = md.type;
+    o.flags = md.flags;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         o.cur[k] = d.cur[k];
@@ -566,16 +589,15 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 #pragma unroll 1
         for (int mi = 0; mi < G; mi++) {
             const int g = g0 + mi;
-            if (g >= p.n_mb) break;
-            const int s = p.mb_stream[g];
-            if (s < 0) continue;
-            const StreamDesc &d = p.streams[s];
-            if (!(d.flags & kStreamActive)) continue;
-            const MbMode md = p.mode[g];
-            const int m = g - d.mb_base;
-            const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[s].pic_quant : d.pic_quant;
-            const int quant = clampi(picq + md.quant_adjust, 1, 31);
-            const bool intra = md.type == H261_MB_TYPE_INTRA;
+            /* the MB as the passes saw it (wave-uniform values from LDS) */
+            const int s = __builtin_amdgcn_readfirstlane(info[mi].s);
+            if (s < 0) continue; /* past the launch, unmapped, or its stream inactive */
+            const int quant = __builtin_amdgcn_readfirstlane(info[mi].quant);
+            const int mtype = __builtin_amdgcn_readfirstlane(info[mi].type);
+            const int mflags = __builtin_amdgcn_readfirstlane(info[mi].flags);
+            const int mvx = __builtin_amdgcn_readfirstlane(info[mi].mvx);
+            const int mvy = __builtin_amdgcn_readfirstlane(info[mi].mvy);
+            const bool intra = mtype == H261_MB_TYPE_INTRA;
             int cbp = 0;
 #pragma unroll
             for (int k = 0; k < 6; k++)
@@ -612,7 +634,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                 if (!intra) bits += c_cbp[cbp].len;
             }
 
-            const int flags = md.flags | (cbp ? H261_MB_COEFF : 0);
+            const int flags = mflags | (cbp ? H261_MB_COEFF : 0);
             if (p.levels) {
                 /* 384 levels as 96 dwords of 4 bytes: AC levels are within
                  * [-127, 127], an intra DC within [1, 255] (read back as uint8) */
@@ -630,24 +652,24 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     o->rgi_tcoeff[k][lane] = intra && lane == 0 ? (int)(uint8_t)mlv[k * 64] : (int)mlv[k * 64 + lane];
                 if (lane == 0) {
                     o->i_macroblock_address = 0;
-                    o->i_macroblock_type = md.type;
+                    o->i_macroblock_type = mtype;
                     o->i_macroblock_type_flags = flags;
                     o->i_macroblock_quant = quant;
-                    o->rgi_mv[0] = md.mvx;
-                    o->rgi_mv[1] = md.mvy;
+                    o->rgi_mv[0] = mvx;
+                    o->rgi_mv[1] = mvy;
                     o->i_coded_block_pattern = cbp;
-                    o->i_mb_x = (m % d.mb_w) * 16;
-                    o->i_mb_y = (m / d.mb_w) * 16;
+                    o->i_mb_x = info[mi].bx;
+                    o->i_mb_y = info[mi].by;
                 }
             }
             if (p.side && lane == 0) {
                 MbSide sd;
-                sd.type = (int8_t)md.type;
+                sd.type = (int8_t)mtype;
                 sd.flags = (int8_t)flags;
                 sd.quant = (int8_t)quant;
                 sd.cbp = (int8_t)cbp;
-                sd.mvx = (int8_t)md.mvx;
-                sd.mvy = (int8_t)md.mvy;
+                sd.mvx = (int8_t)mvx;
+                sd.mvy = (int8_t)mvy;
                 sd.bits = (int16_t)bits;
                 p.side[g] = sd;
             }
