@@ -174,42 +174,25 @@ struct MbInfo {
 };
 static_assert(sizeof(MbInfo) == 128, "four 32-byte rows");
 
-/* MB positions (x | y << 8, in MBs) of the two picture formats H.261 has:
- * CIF (22 x 18 MBs) from entry 0, QCIF (11 x 9) from entry 396 */
-struct MbPos {
-    uint16_t v[396 + 99];
-};
-constexpr MbPos make_mbpos()
-{
-    MbPos t{};
-    for (int m = 0; m < 396; m++) t.v[m] = (uint16_t)(m % 22 | (m / 22) << 8);
-    for (int m = 0; m < 99; m++) t.v[396 + m] = (uint16_t)(m % 11 | (m / 11) << 8);
-    return t;
-}
-__constant__ MbPos c_mbpos = make_mbpos();
-
 __device__ __forceinline__ MbInfo mb_info(const MbLaunch &p, int g)
 {
+    /* loads in two rounds: the MB's stream and mode, then the stream's
+     * descriptor and picture quantiser (indices clamped, checked after) */
     MbInfo o{};
-    o.s = g < p.n_mb ? p.mb_stream[g] : -1;
+    const int gc = g < p.n_mb ? g : p.n_mb - 1;
+    const int s = p.mb_stream[gc];
+    const MbMode md = p.mode[gc];
+    const int sc = s > 0 ? s : 0;
+    const StreamDesc &d = p.streams[sc];
+    const int rcq = p.rc[sc].pic_quant;
+    const uint32_t flags = d.flags;
+    o.s = (g < p.n_mb && s >= 0 && (flags & kStreamActive)) ? s : -1;
     if (o.s < 0) return o;
-    const StreamDesc &d = p.streams[o.s];
-    if (!(d.flags & kStreamActive)) {
-        o.s = -1;
-        return o;
-    }
-    const MbMode md = p.mode[g];
     const int m = g - d.mb_base;
-    if ((d.mb_w == 22 && d.mb_h == 18) || (d.mb_w == 11 && d.mb_h == 9)) {
-        const int pos = c_mbpos.v[(d.mb_w == 22 ? 0 : 396) + m];
-        o.bx = (pos & 0xff) * 16;
-        o.by = (pos >> 8) * 16;
-    } else {
-        o.bx = (m % d.mb_w) * 16;
-        o.by = (m / d.mb_w) * 16;
-    }
+    o.bx = (m % d.mb_w) * 16;
+    o.by = (m / d.mb_w) * 16;
     o.width = d.width;
-    const int picq = (d.flags & kStreamDeviceQuant) ? p.rc[o.s].pic_quant : d.pic_quant;
+    const int picq = (flags & kStreamDeviceQuant) ? rcq : d.pic_quant;
     o.quant = clampi(picq + md.quant_adjust, 1, 31); /* gpu_device.cu:625-627 */
     o.intra = md.type == H261_MB_TYPE_INTRA;
     o.filter = (md.flags & H261_MB_FILTER) != 0;
@@ -223,7 +206,7 @@ __device__ __forceinline__ MbInfo mb_info(const MbLaunch &p, int g)
         o.ref[k] = d.ref[k];
         o.rec[k] = d.rec[k];
     }
-    o.offs = o.intra ? p.denoise[o.s].intra_offset : p.denoise[o.s].inter_offset;
+    o.offs = o.intra ? p.denoise[s].intra_offset : p.denoise[s].inter_offset;
     return o;
 }
 
