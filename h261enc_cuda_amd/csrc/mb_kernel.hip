@@ -256,17 +256,6 @@ __device__ __forceinline__ LaneBlock lane_block(const MbInfo *info, int bi, int 
     return L;
 }
 
-/* stream of the first MB of [g0, g0 + n) that belongs to an active stream,
- * or -1 */
-__device__ __forceinline__ int first_stream(const MbLaunch &p, int g0, int n)
-{
-    for (int i = 0; i < n && g0 + i < p.n_mb; i++) {
-        const int s = p.mb_stream[g0 + i];
-        if (s >= 0 && (p.streams[s].flags & kStreamActive)) return s;
-    }
-    return -1;
-}
-
 /* flushes the wave's denoise statistics of one stream (LDS -> global
  * atomics) and clears them; the frame tail reads them in a later launch */
 __device__ __forceinline__ void flush_stats(const MbLaunch &p, int s, int32_t (*st)[64], int32_t *cnt, int lane)
@@ -320,11 +309,19 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     for (int grp = 0; grp * G < p.mbs_per_wave; grp++) {
         const int g0 = p.mb_first + wg * p.mbs_per_wave + grp * G;
         if (g0 >= p.n_mb) break;
-        const int s0 = first_stream(p, g0, G);
-        if (s0 < 0) continue;
+        /* lane mi < G describes MB g0 + mi; the group's first stream with an
+         * active MB comes from a ballot over those lanes */
+        int my_s = -1;
+        if (lane < G) {
+            const MbInfo o = mb_info(p, g0 + lane);
+            info[lane] = o;
+            my_s = o.s;
+        }
+        const uint64_t has_s = __ballot(my_s >= 0);
+        if (!has_s) continue;
+        const int s0 = __builtin_amdgcn_readlane(my_s, (int)__builtin_ctzll(has_s));
         int s1 = -1; /* the group's other stream, if any */
         uint32_t coded_bits = 0;
-        if (lane < G) info[lane] = mb_info(p, g0 + lane);
         __syncthreads();
 
 #pragma unroll 1
