@@ -171,7 +171,10 @@ struct h261g_engine {
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_variant = kMeTwoStage;
-    int fused_tails = 1;
+    /* 1: rate-control begin in the ME tail; 2: its own launch; 3: its own
+     * launch when a chain fills the device (a short step is latency bound),
+     * else 1; 0: no fused tails */
+    int fused_tails = 3;
     /* bit 0: motion estimation, bit 1: macroblock coder, bit 2: bitstream
      * writer.  Measured at 512 streams: the remap cuts the coder's HBM reads
      * ~9x and its time ~4-14% (at 4 MBs per wave); it cuts motion
@@ -462,7 +465,10 @@ int chain_split(const h261g_engine *e, bool timed)
 int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, bool timed, hipEvent_t *ev)
 {
     const int ns = stream_count(e);
-    const int fused = e->fused_tails;
+    /* rate-control begin in the tail of motion estimation makes each wave
+     * wait for a returning atomic (about 6% of a full launch); as its own
+     * launch it costs a few us, hidden by the other chain */
+    const int fused = e->fused_tails == 3 ? (hi - lo >= kFillWaves ? 2 : 1) : e->fused_tails;
     MeLaunch ml{};
     ml.streams = e->desc;
     ml.mb_stream = e->mb_stream;
@@ -473,7 +479,7 @@ int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, b
     ml.rc = e->rc;
     ml.skip_search_on_intra = 1;
     ml.variant = e->me_variant;
-    ml.fuse_tail = fused;
+    ml.fuse_tail = fused == 1;
     ml.xcd_remap = e->xcd_remap & 1;
     /* one chain: the previous step's bitstreams beside this step's motion
      * estimation (with nothing pending every MB is skipped) */
@@ -487,7 +493,8 @@ int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, b
     }
     if (launch_me(ml, st, timed ? ev[0] : nullptr, timed ? ev[1] : nullptr)) return H261G_ERR_DEVICE;
     if (overlap && !ok(hipStreamWaitEvent(st, e->wr_join, 0), "wait")) return H261G_ERR_DEVICE;
-    if (!fused && launch_rc_begin(e->rc, e->desc, ns, 0, st)) return H261G_ERR_DEVICE;
+    if (fused != 1 && launch_rc_begin(e->rc, e->desc, ns, st, whole ? 0 : lo, whole ? INT32_MAX : hi))
+        return H261G_ERR_DEVICE;
     MbLaunch bl{};
     bl.streams = e->desc;
     bl.mb_stream = e->mb_stream;
@@ -619,7 +626,7 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
         e->mbs_per_wave = value;
         break;
     case H261G_OPT_FUSED_TAILS:
-        if (value != 0 && value != 1) return H261G_ERR_ARG;
+        if (value < 0 || value > 3) return H261G_ERR_ARG;
         e->fused_tails = value;
         break;
     case H261G_OPT_XCD_REMAP:

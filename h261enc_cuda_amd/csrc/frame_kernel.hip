@@ -14,12 +14,12 @@ namespace h261g {
 
 namespace {
 
-__global__ void rc_begin_kernel(RcState *rc_all, const StreamDesc *streams, int n)
+__global__ void rc_begin_kernel(RcState *rc_all, const StreamDesc *streams, int n, int32_t mb_lo, int32_t mb_hi)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const StreamDesc &d = streams[s];
-    if (!(d.flags & kStreamActive)) return;
+    if (!(d.flags & kStreamActive) || d.mb_base < mb_lo || d.mb_base >= mb_hi) return;
     RcState &rc = rc_all[s];
     fc::rc_begin_one(rc, d, rc.cost_accum);
     rc.cost_accum = 0;
@@ -112,11 +112,11 @@ int launch_advance(StreamDesc *streams, SeqState *seq, int n_streams, void *stre
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int, void *stream)
+int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, void *stream, int32_t mb_lo, int32_t mb_hi)
 {
     if (n_streams <= 0) return 0;
     hipLaunchKernelGGL(rc_begin_kernel, dim3((n_streams + 63) / 64), dim3(64), 0, (hipStream_t)stream, rc, streams,
-                       n_streams);
+                       n_streams, mb_lo, mb_hi);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

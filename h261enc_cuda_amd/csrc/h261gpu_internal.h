@@ -170,7 +170,8 @@ struct MeLaunch {
     RcState *rc;        /* optional per-stream: cost accumulation */
     int32_t skip_search_on_intra; /* 1: intra frames only compute the intra cost */
     int32_t variant;              /* kMeTwoStage or kMeSurface */
-    int32_t fuse_tail;            /* 1: the MB that completes a stream runs rate-control begin */
+    int32_t fuse_tail;            /* 1: the MB that completes a stream runs rate-control begin;
+                                     0: costs only accumulate (rate-control begin is launched) */
     int32_t xcd_remap;            /* 1: contiguous macroblock runs per XCD (xcd_logical) */
 };
 enum : int32_t {
@@ -203,9 +204,10 @@ int launch_mb(const MbLaunch &p, void *stream, void *ev_start = nullptr, void *e
  * statistics reset (src/mbgpu_kernels.cuh:478-503) */
 int launch_denoise_update(DenoiseState *dn, const StreamDesc *streams, int n_streams, void *stream);
 
-/* per stream, one thread: picture cost, rate-control picture init */
-int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, int num_mb_unused,
-                    void *stream);
+/* per stream, one thread: picture cost, rate-control picture init (streams
+ * whose MB range starts in [mb_lo, mb_hi)) */
+int launch_rc_begin(RcState *rc, const StreamDesc *streams, int n_streams, void *stream, int32_t mb_lo = 0,
+                    int32_t mb_hi = 0x7fffffff);
 /* per stream, one wave: GOB scan of the MBA / MVD / MQUANT bits, frame
  * size, rate-control update */
 int launch_frame_bits(RcState *rc, const StreamDesc *streams, int n_streams, const MbSide *side,

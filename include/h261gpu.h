@@ -121,8 +121,11 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                     same results) */
 #define H261G_OPT_MBS_PER_WAVE 2 /* macroblocks one wave of the macroblock coder codes
                                     (0: chosen per launch from the batch size) */
-#define H261G_OPT_FUSED_TAILS 3  /* 1 (default): per-stream frame control runs in the tails of the
-                                    two frame-step kernels; 0: as four separate launches */
+#define H261G_OPT_FUSED_TAILS 3  /* 1: per-stream frame control runs in the tails of the two
+                                    frame-step kernels; 2: rate-control begin as its own small
+                                    launch between them, the rest fused; 3 (default): 2 when a
+                                    step (or half of a two-chain step) has enough macroblocks to
+                                    fill the device, else 1; 0: as four separate launches */
 #define H261G_OPT_XCD_REMAP 4    /* bit 0: motion estimation, bit 1: macroblock coder, bit 2:
                                     bitstream writer; a set bit
                                     gives each XCD a contiguous run of macroblocks (neighbours

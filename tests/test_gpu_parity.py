@@ -236,7 +236,7 @@ def test_engine_two_chains_match_one_chain(oracle):
     on its own HIP stream) code every stream as one chain does, stepwise and
     under graph replay, across a close and a re-packing; one stream is also
     checked against the oracle."""
-    from h261enc_cuda_amd._native import OPT_CHAINS
+    from h261enc_cuda_amd._native import OPT_CHAINS, OPT_FUSED_TAILS
     specs = [(CIF, 301, 0, 256000), (QCIF, 302, 0, 64000), (QCIF, 303, 6, 0), (CIF, 304, 0, 384000),
              (QCIF, 305, 0, 128000)]
     n = 10
@@ -244,6 +244,8 @@ def test_engine_two_chains_match_one_chain(oracle):
     for chains in (1, 2):
         eng = Engine(max_streams=8)
         eng.config(OPT_CHAINS, chains)
+        if chains == 2:  # each half's rate-control begin as its own launch, as in full-size steps
+            eng.config(OPT_FUSED_TAILS, 2)
         sids = []
         for size, seed, fixed, rate in specs:
             sid = eng.open(*size, fixed_quant=fixed, bitrate=rate or 512000, vbv_size=rate or 512000)
@@ -289,12 +291,13 @@ def test_engine_two_chains_match_one_chain(oracle):
 
 
 def test_engine_launch_shapes_agree(oracle):
-    """Fused and unfused frame tails, 1/3/8 MBs per wave (workgroups that
-    span several streams) and both search kernels code identical bytes."""
+    """Fused and unfused frame tails, rate-control begin in the ME tail or as
+    its own launch, 1/3/8 MBs per wave (workgroups that span several
+    streams) and both search kernels code identical bytes."""
     from h261enc_cuda_amd._native import OPT_FUSED_TAILS, OPT_MBS_PER_WAVE
     specs = [(QCIF, 301), (CIF, 302), (QCIF, 303), (QCIF, 304), (CIF, 305)]
     n = 5
-    shapes = [(1, 1, 0), (0, 3, 0), (1, 8, 1), (0, 1, 1)]
+    shapes = [(1, 1, 0), (0, 3, 0), (1, 8, 1), (0, 1, 1), (2, 4, 0), (2, 1, 1)]
     results = []
     for fused, mpw, variant in shapes:
         announce(f"launch shape fused={fused} mbs_per_wave={mpw} variant={variant}")
