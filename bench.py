@@ -54,10 +54,10 @@ MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 3
 # dispatch), each scaled by what the same counter reads for the 256 MiB device
 # copy of --copy-probe, divided by the dispatch's MB count
 # (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation
-# 95.6 MB read + 9.7 MB written, the coder 108.0 + 137.4 MB, the bitstream
+# 95.2 MB read + 9.6 MB written, the coder 106.9 + 129.5 MB, the bitstream
 # writer (not remapped) 20.1 + 6.5 MB.  Other remap settings: not measured.
-TRAFFIC_PER_MB = {("me_kernel", True): (95.57e6 + 9.70e6) / 126720,
-                  ("mb_kernel", True): (108.03e6 + 137.40e6) / 126720,
+TRAFFIC_PER_MB = {("me_kernel", True): (95.20e6 + 9.63e6) / 126720,
+                  ("mb_kernel", True): (106.86e6 + 129.53e6) / 126720,
                   ("bitstream_kernel", False): (20.07e6 + 6.47e6) / 126720}
 REMAP_BIT = {"me_kernel": 1, "mb_kernel": 2, "bitstream_kernel": 4}
 
