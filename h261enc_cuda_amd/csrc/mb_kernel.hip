@@ -165,7 +165,7 @@ struct MbInfo {
     const uint8_t *cur[3];
     const uint8_t *ref[3];
     uint8_t *rec[3];
-    const int32_t *offs;   /* the previous frame's denoise offsets of the MB's type */
+    const void *pad0;
     int32_t s;             /* stream; -1: no active stream */
     int32_t bx, by, width; /* luma position and width */
     int32_t quant, intra, filter, mvx, mvy;
@@ -206,7 +206,6 @@ __device__ __forceinline__ MbInfo mb_info(const MbLaunch &p, int g)
         o.ref[k] = d.ref[k];
         o.rec[k] = d.rec[k];
     }
-    o.offs = o.intra ? p.denoise[s].intra_offset : p.denoise[s].inter_offset;
     return o;
 }
 
@@ -271,7 +270,7 @@ __device__ __forceinline__ void flush_stats(const MbLaunch &p, int s, int32_t (*
  * lanes 48-63 idle; G = 4: 24 blocks in three full passes of 8).  Per pass:
  * phase 1 lane (block, row), phase 2 lane (block, column), then lane (block,
  * row) writes an uncoded block's reconstruction or keeps a coded block's
- * prediction.  The group's coded blocks then go through the dequantiser,
+ * prediction in LDS.  The group's coded blocks then go through the dequantiser,
  * IDCT and reconstruction 8 at a time (at the batched bitrates about 9% of
  * inter blocks are coded, so mostly one pass or none).  Last, per MB of the
  * group, the CBP, bit count and outputs. */
@@ -283,12 +282,13 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
     __shared__ float tf[8 * kBlkPitch];
     __shared__ __align__(16) int8_t lv[kBlocks * 64]; /* levels; an intra DC (1..255) as uint8 */
     __shared__ uint2 pred_lds[kBlocks * 8];          /* coded blocks: prediction rows */
-    __shared__ uint4 desc[kBlocks];                  /* coded blocks: reconstruction address, stride,
-                                                        qmul | qadd << 8 | intra << 16 */
     __shared__ int32_t stat[2][2][64]; /* [stream slot][intra, inter][coefficient] */
     __shared__ int32_t scount[2][2];
     __shared__ __align__(16) uint32_t tlen32[kTlenWords];
     __shared__ MbInfo info[G];
+    /* the previous frame's denoise offsets staged as UInt16 (mbgpu:525-528):
+     * [stream slot][intra, inter][coefficient] */
+    __shared__ uint16_t dn_off[2][2][64];
     const uint8_t *tlen = (const uint8_t *)tlen32;
 
     const int lane = threadIdx.x;
@@ -320,7 +320,19 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
         const uint64_t has_s = __ballot(my_s >= 0);
         if (!has_s) continue;
         const int s0 = __builtin_amdgcn_readlane(my_s, (int)__builtin_ctzll(has_s));
-        int s1 = -1; /* the group's other stream, if any */
+        /* the group's other stream, if any (a stream has at least 99 MBs, so
+         * a group of 4 MBs meets at most two) */
+        const uint64_t has_o = __ballot(my_s >= 0 && my_s != s0);
+        const int s1 = has_o ? __builtin_amdgcn_readlane(my_s, (int)__builtin_ctzll(has_o)) : -1;
+        /* lane i loads entry i of the four offset tables now; they go to LDS
+         * after the first pass's phase 1, which hides the loads */
+        uint32_t dnv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int st = k < 2 ? s0 : (s1 >= 0 ? s1 : s0);
+            const int32_t *tab = (k & 1) ? p.denoise[st].inter_offset : p.denoise[st].intra_offset;
+            dnv[k] = (uint32_t)tab[lane];
+        }
         uint32_t coded_bits = 0;
         __syncthreads();
 
@@ -328,21 +340,15 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
         for (int pass = 0; pass < kPasses; pass++) {
             const int bi = pass * 8 + k8; /* block of the group */
             const LaneBlock L = lane_block(info, bi, kBlocks, s0);
-            if (L.act && L.slot) s1 = L.s;
 
             /* ---- phase 1: loads first (current row, reference rows rc-1..rc+1
              * clamped to the block -- the loop filter is 1-D at the block
              * edges; an intra MB has a zero vector -- as aligned dwords, and
              * this lane's eight denoise offsets), then MC / loop filter,
              * residual, DCT rows ---- */
-            uint32_t offp[4]; /* the offsets as UInt16 (mbgpu:525-528), two per register */
             uint32_t predw[2] = {0, 0};
             if (L.act) {
                 const MbInfo &mbi = info[L.mi];
-                const int32_t *offs = mbi.offs;
-                uint32_t offw[8];
-#pragma unroll
-                for (int r = 0; r < 8; r++) offw[r] = (uint32_t)offs[rc + 8 * r];
                 const uint2 cw = *(const uint2 *)(mbi.cur[L.plane] + (size_t)(L.py + rc) * L.stride + L.px);
                 const int rx = L.px + L.mvx;
                 const int a = rx & ~3, o = rx & 3;
@@ -358,8 +364,6 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     rw[k][1] = *(const uint32_t *)(q + a + 4);
                     rw[k][2] = *(const uint32_t *)(q + a2);
                 }
-#pragma unroll
-                for (int r = 0; r < 4; r++) offp[r] = (offw[2 * r] & 0xffffu) | (offw[2 * r + 1] << 16);
                 /* the 8 reference pels of rows rc-1, rc, rc+1 */
                 int a8[3][8];
 #pragma unroll
@@ -414,6 +418,10 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 #pragma unroll
                 for (int k = 0; k < 8; k++) tf[k8 * kBlkPitch + rc * 8 + k] = o8[k];
             }
+            if (pass == 0) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) dn_off[k >> 1][k & 1][lane] = (uint16_t)dnv[k];
+            }
             __syncthreads();
 
             /* ---- phase 2: lane (block, column c) ---- */
@@ -437,7 +445,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 #pragma unroll
                 for (int r = 0; r < 8; r++) {
                     const int i = c + 8 * r;
-                    const int off = (int)((offp[r >> 1] >> (16 * (r & 1))) & 0xffffu);
+                    const int off = dn_off[L.slot][L.t][i];
                     const int cc = coef[r];
                     int av = cc < 0 ? -cc : cc;
                     atomicAdd(&stat[L.slot][L.t][i], av);
@@ -484,11 +492,6 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     *(uint2 *)(dst + (size_t)rc * L.stride) = make_uint2(predw[0], predw[1]);
                 } else {
                     pred_lds[bi * 8 + rc] = make_uint2(predw[0], predw[1]);
-                    if (rc == 0) {
-                        const int qadd = (L.quant - 1) | 1, qmul = L.quant * 2;
-                        desc[bi] = make_uint4((uint32_t)(uintptr_t)dst, (uint32_t)((uintptr_t)dst >> 32),
-                                              (uint32_t)L.stride, (uint32_t)(qmul | qadd << 8 | (L.intra ? 1 << 16 : 0)));
-                    }
                 }
             }
             __syncthreads(); /* tf is reused by the next pass */
@@ -508,9 +511,10 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                 rest &= rest - 1;
             }
             const bool act = bi >= 0;
-            const uint4 de = desc[act ? bi : 0];
-            const bool intra = (de.w >> 16) & 1;
-            const int qmul = (int)(de.w & 0xff), qadd = (int)((de.w >> 8) & 0xff);
+            /* the block's MB descriptor gives its quantiser, type and place */
+            const LaneBlock C = lane_block(info, act ? bi : 0, kBlocks, s0);
+            const bool intra = C.intra;
+            const int qmul = C.quant * 2, qadd = (C.quant - 1) | 1;
             if (act) {
                 const int c = rc;
                 /* dequantiser (mbgpu:763-814) */
@@ -558,8 +562,8 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     const int val = clampi(pv + rres, 0, 255);
                     w[x >> 2] |= (uint32_t)val << (8 * (x & 3));
                 }
-                uint8_t *dst = (uint8_t *)(((uintptr_t)de.y << 32) | de.x);
-                *(uint2 *)(dst + (size_t)rc * de.z) = make_uint2(w[0], w[1]);
+                uint8_t *dst = info[C.mi].rec[C.plane] + (size_t)(C.py + rc) * C.stride + C.px;
+                *(uint2 *)dst = make_uint2(w[0], w[1]);
             }
             __syncthreads(); /* tf and slots are reused by the next pass */
         }
