@@ -462,7 +462,8 @@ int chain_split(const h261g_engine *e, bool timed)
 }
 
 /* one frame step of the streams whose MBs lie in [lo, hi), on stream st */
-int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, bool timed, hipEvent_t *ev)
+int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, bool timed, hipEvent_t *ev,
+                  hipEvent_t me_done = nullptr)
 {
     const int ns = stream_count(e);
     /* rate-control begin in the tail of motion estimation makes each wave
@@ -492,6 +493,7 @@ int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, b
             return H261G_ERR_DEVICE;
     }
     if (launch_me(ml, st, timed ? ev[0] : nullptr, timed ? ev[1] : nullptr)) return H261G_ERR_DEVICE;
+    if (me_done && !ok(hipEventRecord(me_done, st), "event")) return H261G_ERR_DEVICE;
     if (overlap && !ok(hipStreamWaitEvent(st, e->wr_join, 0), "wait")) return H261G_ERR_DEVICE;
     if (fused != 1 && launch_rc_begin(e->rc, e->desc, ns, st, whole ? 0 : lo, whole ? INT32_MAX : hi))
         return H261G_ERR_DEVICE;
@@ -562,10 +564,13 @@ int enqueue_steps(h261g_engine *e, int k, bool timed)
         }
         return H261G_OK;
     }
-    if (!ok(hipEventRecord(e->chain_fork, st), "event") || !ok(hipStreamWaitEvent(e->chain_stream, e->chain_fork, 0), "wait"))
-        return H261G_ERR_DEVICE;
+    /* the second chain starts after the first chain's first motion
+     * estimation: the halves then run about half a step apart, so one half's
+     * coder and writer overlap the other half's motion estimation */
     for (int i = 0; i < k; i++) {
-        int rcode = enqueue_chain(e, st, 0, split, false, false, nullptr);
+        int rcode = enqueue_chain(e, st, 0, split, false, false, nullptr, i == 0 ? e->chain_fork : nullptr);
+        if (!rcode && i == 0 && !ok(hipStreamWaitEvent(e->chain_stream, e->chain_fork, 0), "wait"))
+            rcode = H261G_ERR_DEVICE;
         if (!rcode) rcode = enqueue_chain(e, e->chain_stream, split, e->mb_used, false, false, nullptr);
         if (rcode) return rcode;
     }
