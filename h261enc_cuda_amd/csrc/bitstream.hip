@@ -30,7 +30,54 @@ namespace h261g {
 namespace {
 
 __constant__ uint8_t c_zz[64] = H261G_ZIGZAG;
-__constant__ Vlc c_tc[kTcoeffRuns][kTcoeffLevels] = H261G_TCOEFF_TABLE;
+/* The run/level table compacted: word 0 has length 0 (no code); then, run
+ * by run, the codes of levels 1..max as code | len << 16; then, per run,
+ * its first code's word | its largest level << 8.  About 90 words instead
+ * of the 432 of the full table, so less LDS per wave. */
+constexpr Vlc kTcFull[kTcoeffRuns][kTcoeffLevels] = H261G_TCOEFF_TABLE;
+constexpr int tc_codes()
+{
+    int n = 0;
+    for (int r = 0; r < kTcoeffRuns; r++)
+        for (int l = 1; l < kTcoeffLevels; l++) n += kTcFull[r][l].len ? 1 : 0;
+    return n;
+}
+constexpr int kTcCodes = tc_codes();
+constexpr int kTcRunBase = 1 + kTcCodes;
+constexpr int kTc = kTcRunBase + kTcoeffRuns;
+struct TcCompact {
+    uint32_t w[kTc];
+};
+/* a run's codes must be levels 1..max without gaps for the compact form */
+constexpr bool tc_contiguous()
+{
+    for (int r = 0; r < kTcoeffRuns; r++) {
+        bool gap = false;
+        for (int l = 1; l < kTcoeffLevels; l++) {
+            if (!kTcFull[r][l].len) gap = true;
+            else if (gap) return false;
+        }
+    }
+    return kTcFull[0][0].len == 0;
+}
+static_assert(tc_contiguous(), "TCOEFF codes of a run cover levels 1..max");
+constexpr TcCompact make_tc()
+{
+    TcCompact t{};
+    int k = 1;
+    for (int r = 0; r < kTcoeffRuns; r++) {
+        const int first = k;
+        int maxl = 0;
+        for (int l = 1; l < kTcoeffLevels; l++)
+            if (kTcFull[r][l].len) {
+                t.w[k++] = (uint32_t)kTcFull[r][l].code | ((uint32_t)kTcFull[r][l].len << 16);
+                maxl = l;
+            }
+        t.w[kTcRunBase + r] = (uint32_t)first | ((uint32_t)maxl << 8);
+    }
+    return t;
+}
+__constant__ TcCompact c_tc = make_tc();
 
 /* LDS words of one MB's bits: its header (<= 64 bits), 6 blocks of at most
  * 8 + 63 * 20 + 2 bits, and the leading partial word */
@@ -45,11 +92,10 @@ constexpr int kPieceRounds = (kPieces + 63) / 64;
  * at 512 streams: 1 wave 38 us, 2 or 4 waves 40 us, 8 waves 54 us per
  * launch at full load) */
 constexpr int kWavesPerGroup = 1;
-constexpr int kTc = kTcoeffRuns * kTcoeffLevels;
 constexpr int kTcRounds = (kTc + 64 * kWavesPerGroup - 1) / (64 * kWavesPerGroup);
 /* non-zero coefficients buffered before they are coded (a block is never
  * split between two flushes) */
-constexpr int kEntryCap = 256;
+constexpr int kEntryCap = 128;
 
 /* a buffered coefficient: transmission position, level byte and flags */
 enum : uint32_t {
@@ -133,8 +179,10 @@ __device__ __forceinline__ void flush_entries(WaveLds &w, const uint32_t *tc, in
         const int l = dc ? (int)((e >> 16) & 0xffu) : (int)(int8_t)((e >> 16) & 0xffu);
         const int a = l < 0 ? -l : l;
         const uint32_t sign = l < 0 ? 1u : 0u;
-        const bool in_range = run >= 0 && run < kTcoeffRuns && a < kTcoeffLevels;
-        const uint32_t te = tc[in_range ? run * kTcoeffLevels + a : 0]; /* entry 0 has length 0 */
+        const bool run_ok = run >= 0 && run < kTcoeffRuns;
+        const uint32_t ri = tc[kTcRunBase + (run_ok ? run : 0)];
+        const bool in_range = run_ok && a >= 1 && a <= (int)(ri >> 8);
+        const uint32_t te = tc[in_range ? (int)(ri & 0xffu) + a - 1 : 0]; /* word 0 has length 0 */
         const int tlen = (int)((te >> 16) & 0xffu);
         /* escape (000001, 6-bit run, 8-bit level), else the table code and
          * its sign, else an inter block's first coefficient +-1 at position
@@ -171,7 +219,7 @@ __device__ __forceinline__ void flush_entries(WaveLds &w, const uint32_t *tc, in
 __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunch p)
 {
     __shared__ WaveLds lds_all[kWavesPerGroup];
-    __shared__ uint32_t tc[kTc]; /* run/level table: code | len << 16 */
+    __shared__ uint32_t tc[kTc]; /* the compacted run/level table */
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int n_waves = (p.n_mb - p.mb_first + kMbsPerWave - 1) / kMbsPerWave;
     const int n_groups = (n_waves + kWavesPerGroup - 1) / kWavesPerGroup;
@@ -183,8 +231,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
     /* every load is issued before its value is needed: two rounds of
      * memory latency per wave (places and side information, then levels
      * and bitstream words), none inside the coding loop */
-    static_assert(sizeof(Vlc) == 4, "one word per table entry");
-    const uint32_t *tcg = (const uint32_t *)&c_tc[0][0];
+    const uint32_t *tcg = c_tc.w;
     uint32_t tcv[kTcRounds];
 #pragma unroll
     for (int i = 0; i < kTcRounds; i++) {
