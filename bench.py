@@ -54,11 +54,11 @@ MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 3
 # dispatch), each scaled by what the same counter reads for the 256 MiB device
 # copy of --copy-probe, divided by the dispatch's MB count
 # (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation
-# 95.2 MB read + 9.6 MB written, the coder 106.9 + 129.5 MB, the bitstream
+# 95.2 MB read + 9.6 MB written, the coder 106.9 + 127.9 MB, the bitstream
 # writer (not remapped) 20.1 + 6.5 MB.  Other remap settings: not measured.
 TRAFFIC_PER_MB = {("me_kernel", True): (95.20e6 + 9.63e6) / 126720,
-                  ("mb_kernel", True): (106.86e6 + 129.53e6) / 126720,
-                  ("bitstream_kernel", False): (20.07e6 + 6.47e6) / 126720}
+                  ("mb_kernel", True): (106.91e6 + 127.91e6) / 126720,
+                  ("bitstream_kernel", False): (20.10e6 + 6.47e6) / 126720}
 REMAP_BIT = {"me_kernel": 1, "mb_kernel": 2, "bitstream_kernel": 4}
 
 
