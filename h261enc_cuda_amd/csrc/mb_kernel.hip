@@ -23,10 +23,13 @@
  * unsuffixed literal are double multiplies rounded to float; the forward
  * DCT rounds with floorf((float)(x + 0.49999)), the IDCT with
  * floor((double)x + 0.49999) (src/mbgpu_kernels.cuh:337-442, 827-931).
- * The denoise uses the previous frame's offsets and accumulates this
- * frame's |coefficient| statistics (LDS atomics, one global flush per
- * workgroup); the CBP is the OR of this frame's coded blocks, the intended
- * result of the reference's racy reset/atomicOr (mbgpu:676-721).
+ * A wave codes groups of consecutive MBs; per group, each MB's stream,
+ * position, quantiser, mode and plane pointers, and the denoise offset
+ * tables of the group's streams, are read once into LDS.  The denoise uses
+ * the previous frame's offsets and accumulates this frame's |coefficient|
+ * statistics (LDS atomics, one global flush per group); the CBP is the OR
+ * of this frame's coded blocks, the intended result of the reference's racy
+ * reset/atomicOr (mbgpu:676-721).
  */
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
@@ -165,12 +168,11 @@ struct MbInfo {
     const uint8_t *cur[3];
     const uint8_t *ref[3];
     uint8_t *rec[3];
-    const void *pad0;
     int32_t s;             /* stream; -1: no active stream */
     int32_t bx, by, width; /* luma position and width */
     int32_t quant, intra, filter, mvx, mvy;
-    int32_t type, flags; /* the mode's MB type and flags */
-    int32_t pad;
+    int32_t type, flags;   /* the mode's MB type and flags */
+    int32_t pad[3];
 };
 static_assert(sizeof(MbInfo) == 128, "four 32-byte rows");
 

@@ -8,7 +8,10 @@
  *      {-14,-11,-8,-5,-2,0,2,5,8,11,14}^2 (src/megpu_kernels.cuh:29-32),
  *      stage 2 the 7x7 refine around the winner (megpu:309-312, 338-339),
  *      each candidate one lane, 16 rows x 4 packed v_sad_u8 (alignbyte
- *      realigns the window bytes to the candidate's column);
+ *      realigns the window bytes to the candidate's column); which lane
+ *      takes which candidate comes from constant tables laid out so that a
+ *      half-wave's LDS reads never meet in a bank, and validity is a lane
+ *      mask built with scalar instructions;
  *   3. the winner is the wave minimum of (SAD << 8 | morton(tx, ty)):
  *      the reference's quad-tree reduction with strict '<' in the order
  *      self, +x, +y, +xy (megpu:95-154, 375-434) selects exactly the
