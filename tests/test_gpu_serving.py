@@ -25,13 +25,20 @@ def alone(r: Request):
     return out
 
 
-@pytest.mark.parametrize("async_input", [True, False], ids=["async-input", "sync-input"])
-def test_serving_mix_equals_alone(oracle, async_input):
+@pytest.mark.parametrize("async_input,chains", [(True, 0), (False, 0), (True, 2)],
+                         ids=["async-input", "sync-input", "async-input-two-chains"])
+def test_serving_mix_equals_alone(oracle, async_input, chains):
     reqs = poisson_requests(rate=400.0, duration=0.05, seed=11, median_frames=6, sigma=0.5, cap=12)
     assert len(reqs) >= 8
-    announce(f"serving {len(reqs)} requests, async input {async_input}")
+    announce(f"serving {len(reqs)} requests, async input {async_input}, chains {chains}")
     # a tight budget forces admission to wait for memory
     loop = ServingLoop(hbm_budget=6 * (1 << 20) * 3, max_streams=16, async_input=async_input)
+    if chains:
+        # the halves of the MB space as separate chains, with rate-control
+        # begin launched, as in steps that fill the device
+        from h261enc_cuda_amd import _native as nat
+        loop.eng.config(nat.OPT_CHAINS, chains)
+        loop.eng.config(nat.OPT_FUSED_TAILS, 2)
     stats, outs = loop.run(reqs, collect=True, realtime=False)
     loop.close()
     assert stats.frames == sum(r.frames for r in reqs)
