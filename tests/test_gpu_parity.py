@@ -154,6 +154,49 @@ def test_engine_single_stream_matches_oracle(oracle, size, fixed, variant):
     eng.destroy()
 
 
+def extreme_frames(size, rng):
+    """Full-swing content: flat 0 and 255 frames, 0/255 checkerboards and
+    stripes and their inversions, uniform noise.  Residuals reach +-255, so
+    the quantiser's +-127 clamp, the dequantiser's [-2048, 2047] clip, the
+    IDCT's [-256, 255] clamp and the reconstruction's [0, 255] clamp all
+    act; SADs reach their 16x16 maximum of 65,280."""
+    w, h = size
+    shapes = ((h, w), (h // 2, w // 2), (h // 2, w // 2))
+    yy, xx = np.mgrid[0:h, 0:w]
+    checker = (((yy >> 3) + (xx >> 3)) & 1).astype(np.uint8) * 255
+    stripes = ((xx & 1) * 255).astype(np.uint8)
+
+    def planes(y):
+        return (y, y[::2, ::2].copy(), (255 - y[::2, ::2]).copy())
+
+    flat0 = tuple(np.zeros(s, np.uint8) for s in shapes)
+    flat255 = tuple(np.full(s, 255, np.uint8) for s in shapes)
+    noise = tuple(rng.integers(0, 256, s, dtype=np.uint8) for s in shapes)
+    shifted = tuple(np.roll(p, (5, -7) if i == 0 else (2, -3), axis=(0, 1)) for i, p in enumerate(noise))
+    return [flat0, flat255, planes(checker), planes(255 - checker), noise, shifted, planes(stripes), flat0]
+
+
+@pytest.mark.parametrize("variant", [0, 1], ids=["two-stage", "surface"])
+@pytest.mark.parametrize("size,fixed", [(CIF, 1), (QCIF, 31), (CIF, 0)], ids=["cif-q1", "qcif-q31", "cif-rc"])
+def test_engine_extreme_content_matches_oracle(oracle, size, fixed, variant):
+    w, h = size
+    frames = extreme_frames(size, np.random.default_rng(97))
+    eng = Engine(max_streams=2, me_variant=variant)
+    sid = eng.open(w, h, fixed_quant=fixed)
+    announce(f"engine extreme content {size} fixed={fixed} variant={variant}")
+    res = run_engine_host_fed(eng, sid, frames)
+    os_ = oracle.stream(w, h, fixed_quant=fixed)
+    for t, f in enumerate(frames):
+        ob, orec, _, _, oq = os_.encode(*f, want_details=True)
+        bs, rec, q, nbytes, _ = res[t]
+        assert q == oq, f"frame {t}: quantiser {q} vs {oq}"
+        assert nbytes == len(ob), f"frame {t}: device frame size {nbytes} vs {len(ob)}"
+        assert bs == ob, f"frame {t}: bitstream differs"
+        for p in range(3):
+            assert np.array_equal(rec[p], orec[p]), f"frame {t}: recon plane {p} differs"
+    eng.destroy()
+
+
 def test_engine_batched_matches_oracle_and_alone(oracle):
     """Mixed CIF/QCIF streams with device-generated input, rate control on,
     coded together: every stream equals the oracle, and equals itself coded
