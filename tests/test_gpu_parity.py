@@ -65,12 +65,13 @@ def test_legacy_me_matches_oracle(oracle, size, variant, monkeypatch):
         dev.close()
 
 
-@pytest.mark.parametrize("size,fixed", [(CIF, 8), (QCIF, 0)], ids=["cif-q8", "qcif-rc"])
-def test_legacy_encode_matches_oracle(oracle, size, fixed):
+@pytest.mark.parametrize("size,fixed,full_swing", [(CIF, 8, False), (QCIF, 0, False), (CIF, 1, True)],
+                         ids=["cif-q8", "qcif-rc", "cif-q1-full-swing"])
+def test_legacy_encode_matches_oracle(oracle, size, fixed, full_swing):
     """encode/decode through the drop-in entry points with the oracle's mode
     decision; macroblock structs and reconstructions compared per frame."""
     w, h = size
-    frames = seq(21, size, 6, sigma=2, motion=7)
+    frames = extreme_frames(size, np.random.default_rng(98)) if full_swing else seq(21, size, 6, sigma=2, motion=7)
     os_ = oracle.stream(w, h, fixed_quant=fixed)
     dev = LegacyDevice(w, h)
     ref = zeros(size)
@@ -156,10 +157,9 @@ def test_engine_single_stream_matches_oracle(oracle, size, fixed, variant):
 
 def extreme_frames(size, rng):
     """Full-swing content: flat 0 and 255 frames, 0/255 checkerboards and
-    stripes and their inversions, uniform noise.  Residuals reach +-255, so
-    the quantiser's +-127 clamp, the dequantiser's [-2048, 2047] clip, the
-    IDCT's [-256, 255] clamp and the reconstruction's [0, 255] clamp all
-    act; SADs reach their 16x16 maximum of 65,280."""
+    stripes and their inversions, uniform noise.  Residuals reach +-255; at
+    quantiser 1 about 11,000 AC levels per sequence sit at the +-127 clamp,
+    SATD costs reach about 110,000 and CIF frames reach 320 KB."""
     w, h = size
     shapes = ((h, w), (h // 2, w // 2), (h // 2, w // 2))
     yy, xx = np.mgrid[0:h, 0:w]
