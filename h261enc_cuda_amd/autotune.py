@@ -78,7 +78,14 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
             return ms
 
         default_remap = 3
-        best_mpw = min(mpw_c, key=lambda m: trial(m, default_remap))
+        # within the tie band a fixed count wins over 0 ("by batch size"):
+        # the sample is at full load, where 0 picks the same shape, but as
+        # streams finish 0 changes shape and a job ran about 7% slower
+        mpw_t = {m: trial(m, default_remap) for m in mpw_c}
+        mpw_fast = min(mpw_t.values())
+        mpw_close = [m for m in mpw_c if mpw_t[m] <= mpw_fast * (1 + tie)]
+        best_mpw = min((m for m in mpw_close if m), key=lambda m: mpw_t[m],
+                       default=min(mpw_close, key=lambda m: mpw_t[m]))
         times = {r: trial(best_mpw, r) for r in remap_c}
         fastest = min(times.values())
         close = [r for r in remap_c if times[r] <= fastest * (1 + tie)]
