@@ -33,6 +33,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from h261enc_cuda_amd.workloads import batched_streams, hard_streams, open_streams, run_job  # noqa: E402
+
 METRIC = "CIF frames/s on 1 MI355X, batched vs plain, bit-exact; p99 frame latency"
 HBM_PEAK_GBS = 8000.0
 SAD_PEAK = 157.3e12  # byte-SADs/s: 256 CU x 64 lanes x 2.4 GHz x 4 (v_sad_u8)
@@ -86,73 +88,6 @@ def coarse_valid_candidates(mb_w, mb_h):
 # candidates per MB, 256 byte-SADs each (an upper bound for the refine part)
 SAD_OPS = {396: (coarse_valid_candidates(22, 18) + 49 * 396) * 256,
            99: (coarse_valid_candidates(11, 9) + 49 * 99) * 256}
-
-
-def batched_streams(seed=2026):
-    rng = np.random.default_rng(seed)
-    specs = []
-    for i in range(512):
-        cif = i < 256
-        specs.append(dict(size=(352, 288) if cif else (176, 144), length=int(rng.integers(60, 601)),
-                          bitrate=int(rng.integers(64, 385)) * 1000, seed=int(rng.integers(1, 2 ** 62)),
-                          motion=15))
-    return specs
-
-
-def hard_streams(seed=2027):
-    """1024 mixed QCIF/CIF streams, per-macroblock local motion (random
-    affine warp + 10% random vectors), scene cuts every 50 frames, sigma 4"""
-    rng = np.random.default_rng(seed)
-    specs = []
-    for i in range(1024):
-        cif = i < 512
-        specs.append(dict(size=(352, 288) if cif else (176, 144), length=int(rng.integers(60, 601)),
-                          bitrate=int(rng.integers(64, 385)) * 1000, seed=int(rng.integers(1, 2 ** 62)),
-                          motion=0, local=True, scene_cut=50, sigma=4))
-    return specs
-
-
-def open_streams(eng, specs, sigma=2, frames=None):
-    ids = []
-    for s in specs:
-        sid = eng.open(*s["size"], bitrate=s["bitrate"], vbv_size=s["bitrate"], init_quant=10,
-                       fixed_quant=s.get("fixed", 0))
-        n = s["length"] if frames is None else min(frames, s["length"])
-        eng.generate_input(sid, s["seed"], n, sigma=s.get("sigma", sigma), motion=s["motion"],
-                           local=s.get("local", False), scene_cut=s.get("scene_cut", 0))
-        ids.append(sid)
-    eng.sync()
-    return ids
-
-
-def run_job(eng, ids, specs, steps, start=0, compact_every=30, after_chunk=None):
-    """frame steps start..start+steps-1 (graph replay; eager with events when
-    the engine profiles), closing finished streams and re-packing the MB
-    space every compact_every steps (closing and re-packing are queued
-    behind the steps: nothing waits).  Returns (seconds, frames, mbs)."""
-    lengths = np.array([s["length"] for s in specs])
-    nmb = np.array([(s["size"][0] // 16) * (s["size"][1] // 16) for s in specs])
-    done = start
-    end = start + steps
-    open_mask = lengths > start
-    eng.sync()
-    t0 = time.perf_counter()
-    while done < end:
-        chunk = min(compact_every, end - done)
-        eng.run(chunk)
-        done += chunk
-        if after_chunk is not None:
-            after_chunk(done)
-        fin = open_mask & (lengths <= done)
-        if fin.any() and done < end:
-            for i in np.nonzero(fin)[0]:
-                eng.close(ids[i])
-            open_mask &= ~fin
-            eng.compact()
-    eng.sync()
-    dt = time.perf_counter() - t0
-    frames = np.minimum(lengths, end) - np.minimum(lengths, start)
-    return dt, int(frames.sum()), int((frames * nmb).sum())
 
 
 def algorithmic(specs, t0, t1):

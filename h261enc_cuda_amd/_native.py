@@ -12,6 +12,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libh261gpu.so")
+# the host content generator alone: no HIP runtime, so CPU-only processes
+# (checker pools, file writers) can make input frames without touching the GPU
+CONTENT_LIB_PATH = os.path.join(_HERE, "libh261content.so")
 
 # layouts of the ABI structs (include/h261gpu_abi.h), little endian
 ME_MB_DTYPE = np.dtype(
@@ -155,6 +158,26 @@ def load() -> ctypes.CDLL:
         fn.restype = res
         fn.argtypes = args
     _lib = lib
+    return lib
+
+
+_content_lib = None
+
+
+def load_content() -> ctypes.CDLL:
+    """Loads libh261content.so (h261g_content_frame_host only); raises if it
+    has not been built."""
+    global _content_lib
+    if _content_lib is not None:
+        return _content_lib
+    if not os.path.exists(CONTENT_LIB_PATH):
+        raise RuntimeError(
+            f"{CONTENT_LIB_PATH} is missing: build it with __graft_entry__.build() (or make -C h261enc_cuda_amd)")
+    lib = ctypes.CDLL(CONTENT_LIB_PATH)
+    lib.h261g_content_frame_host.restype = ctypes.c_int
+    lib.h261g_content_frame_host.argtypes = [ctypes.POINTER(ContentParams), ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    _content_lib = lib
     return lib
 
 

@@ -15,8 +15,8 @@ mix of stream sizes, all streams active, inter frames -- with HIP events
 around every launch of the step, and keeps the fastest: first over
 `mbs_per_wave` at the default remap, then over the remap at the best
 `mbs_per_wave` (coordinate descent; the two knobs act on mostly different
-kernels).  Among remaps within `tie` (1%) of the fastest, one that remaps
-motion estimation is preferred: its time is the same within noise, and it
+kernels).  Among remaps within `tie` (1%) of the fastest, the fastest that
+remaps motion estimation is preferred: its time is the same within noise, and it
 reads about 7x fewer HBM bytes (neighbouring MBs' search windows meet in one
 XCD's L2), which leaves the memory system to concurrent work.  One engine
 serves every trial; each trial first runs one untimed step with its
@@ -89,7 +89,8 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
         times = {r: trial(best_mpw, r) for r in remap_c}
         fastest = min(times.values())
         close = [r for r in remap_c if times[r] <= fastest * (1 + tie)]
-        best_remap = next((r for r in close if r & 1), min(close, key=lambda r: times[r]))
+        best_remap = min((r for r in close if r & 1), key=lambda r: times[r],
+                         default=min(close, key=lambda r: times[r]))
     finally:
         eng.destroy()
     best = min((t for t in trials if t["mbs_per_wave"] == best_mpw and t["xcd_remap"] == best_remap),

@@ -221,6 +221,9 @@ struct h261g_engine {
     hipGraph_t graph = nullptr, graph_k = nullptr;
     hipGraphExec_t graph_exec = nullptr, graph_exec_k = nullptr;
     bool graph_valid = false;
+    /* the captured step leaves its bitstream writer to the next step (one
+     * overlapped chain): every replay ends with that writer pending */
+    bool graph_leaves_writer = false;
     /* profiling */
     bool profile = false;
     /* profiling: a ring of event sets, one per step, read back lazily so the
@@ -987,6 +990,9 @@ int h261g_engine_compact(h261g_engine_t *e)
         if (!s.open) continue;
         s.mb_base = used;
         s.desc0.mb_base = used;
+        /* an input still generating writes gen_d whole when it activates:
+         * it must carry the new base, which remap_kernel gives the device */
+        s.gen_d.mb_base = used;
         bases[i] = used;
         used += s.n_mb;
     }
@@ -1043,7 +1049,8 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     if (s->fresh) {
         d = s->desc0;
         q = s->seq0;
-    } else if (!ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy") ||
+    } else if (!ok(hipStreamSynchronize(e->stream), "sync") ||
+               !ok(hipMemcpy(&d, e->desc + id, sizeof(d), hipMemcpyDeviceToHost), "hipMemcpy") ||
                !ok(hipMemcpy(&q, e->seq + id, sizeof(q), hipMemcpyDeviceToHost), "hipMemcpy")) {
         return H261G_ERR_DEVICE;
     }
@@ -1126,6 +1133,10 @@ int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
     }
     if (chain_split(e, false) && flush_writer(e)) return H261G_ERR_DEVICE;
     if (!e->graph_valid) {
+        /* capture queues nothing, so a writer left by earlier steps is
+         * launched now; the captured step's own writer flag is recorded and
+         * applied after each replay below */
+        if (flush_writer(e)) return H261G_ERR_DEVICE;
         if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
         if (e->graph) (void)hipGraphDestroy(e->graph);
         if (e->graph_exec_k) (void)hipGraphExecDestroy(e->graph_exec_k);
@@ -1142,15 +1153,23 @@ int h261g_engine_run(h261g_engine_t *e, int32_t n_steps)
             if (rcode || !ok(ce, "end capture")) return H261G_ERR_DEVICE;
             if (!ok(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "instantiate")) return H261G_ERR_DEVICE;
         }
+        /* enqueue_steps set the flag as if the captured steps had run */
+        e->graph_leaves_writer = e->writer_pending;
+        e->writer_pending = false;
         e->graph_valid = true;
     }
+    /* a replayed step launches a pending writer first only when it was
+     * captured with one (the same overlapped shape) */
+    if (n_steps > 0 && !e->graph_leaves_writer && flush_writer(e)) return H261G_ERR_DEVICE;
     int left = n_steps;
     for (; left >= h261g_engine::kGraphSteps; left -= h261g_engine::kGraphSteps)
         if (!ok(hipGraphLaunch(e->graph_exec_k, e->stream), "graph launch")) return H261G_ERR_DEVICE;
     for (; left > 0; left--)
         if (!ok(hipGraphLaunch(e->graph_exec, e->stream), "graph launch")) return H261G_ERR_DEVICE;
-    if (n_steps > 0)
+    if (n_steps > 0) {
         for (auto &s : e->slots) s.fresh = false;
+        e->writer_pending = e->graph_leaves_writer;
+    }
     for (auto &s : e->slots) s.pending = false;
     return H261G_OK;
 }

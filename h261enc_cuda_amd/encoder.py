@@ -22,8 +22,10 @@ QCIF = (176, 144)
 
 def content_frame(seed: int, width: int, height: int, frame: int, sigma: int = 2, motion: int = 7,
                   local: bool = False, scene_cut: int = 0) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
-    """Frame `frame` of the synthetic sequence (Y, Cb, Cr as uint8 arrays)."""
-    lib = nat.load()
+    """Frame `frame` of the synthetic sequence (Y, Cb, Cr as uint8 arrays),
+    made on the host (libh261content.so; the device generator makes the same
+    pels)."""
+    lib = nat.load_content()
     cp = nat.ContentParams(seed=seed, sigma=sigma, motion=motion, local=int(local), scene_cut=scene_cut)
     y = np.zeros((height, width), np.uint8)
     cb = np.zeros((height // 2, width // 2), np.uint8)

@@ -181,13 +181,15 @@ H261G_API int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_
  * content into HBM; the stream then codes one per step until the sequence
  * ends (the stream stays open, inactive, afterwards) */
 H261G_API int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_content_params_t *c, int32_t n_frames);
-/* number of frames a stream has coded (synchronises) */
 /* 1 while a stream's asynchronous input (H261G_OPT_ASYNC_INPUT) is still
  * being generated, 0 once it takes part in frame steps (from the step call
  * that activated it); host state only, no device access */
 H261G_API int h261g_stream_input_pending(h261g_engine_t *e, int32_t id);
+/* number of frames a stream has coded (synchronises) */
 H261G_API int h261g_stream_frames_done(h261g_engine_t *e, int32_t id);
-/* re-packs the macroblock index space after streams closed (synchronises) */
+/* re-packs the macroblock index space after streams closed: the new stream
+ * bases are queued behind the work queued so far and the call returns at
+ * once (it waits only for the previous re-packing's staging copy) */
 H261G_API int h261g_engine_compact(h261g_engine_t *e);
 
 /* encodes one frame for every stream that has an input; asynchronous.

@@ -8,8 +8,8 @@
  * src/h261_gob.c, src/h261_macroblock.c, src/h261_main.c) reads and writes
  * them directly and passes pointers to them across the device C ABI.
  * Field order, field types and therefore offsets are fixed by that
- * contract; tests/test_abi.py compares every size and offset against the
- * reference headers compiled by gcc.
+ * contract; tests/test_oracle_reference.py compares every size and offset
+ * against the reference headers compiled by gcc.
  *
  * Sizes on x86-64 (gcc, LP64):
  *   me_gpu_mv_t 2, me_gpu_mb_t 20, me_gpu_t 88, h261_mb_mode_decision_t 24,

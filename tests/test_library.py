@@ -85,3 +85,29 @@ def test_xcd_remap_is_a_bijection(tmp_path):
         # dealt to that XCD
         for i, l in enumerate(idx):
             assert (l // chunk) % 8 == i % 8, (n, i)
+
+
+def test_host_content_library_needs_no_hip_runtime():
+    """libh261content.so (the host content generator for CPU-only processes)
+    exports h261g_content_frame_host and links no HIP library; it makes the
+    same frames as libh261gpu.so's copy."""
+    import ctypes
+    import numpy as np
+    path = _native.CONTENT_LIB_PATH
+    deps = subprocess.run(["ldd", path], check=True, capture_output=True, text=True).stdout
+    assert "amdhip" not in deps and "hsa" not in deps, deps
+    a = _native.load_content()
+    b = _native.load()
+    cp = _native.ContentParams(seed=77, sigma=3, motion=9, local=0, scene_cut=0)
+    for lib in (a, b):
+        lib.h261g_content_frame_host.restype = ctypes.c_int
+    out = []
+    for lib in (a, b):
+        y = np.zeros((144, 176), np.uint8)
+        c = np.zeros((72, 88), np.uint8)
+        r = np.zeros((72, 88), np.uint8)
+        assert lib.h261g_content_frame_host(ctypes.byref(cp), 176, 144, 5, _native.ptr(y), _native.ptr(c),
+                                            _native.ptr(r)) == 0
+        out.append((y, c, r))
+    for p in range(3):
+        assert np.array_equal(out[0][p], out[1][p])
