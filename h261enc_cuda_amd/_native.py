@@ -54,8 +54,9 @@ EXPORTS = [
     "h261g_stream_read_me", "h261g_stream_read_bitstream", "h261g_stream_read_input",
     "h261g_engine_profile", "h261g_engine_kernel_times", "h261g_content_frame_host", "h261g_write_picture",
     "h261g_engine_config", "h261g_device_info", "h261g_stream_input_pending", "h261g_engine_fetch_bitstreams",
-    "h261g_engine_fetched",
+    "h261g_engine_fetched", "h261g_engine_fetch_bitstreams_into",
 ]
+FETCH_INTO_MAX_STREAMS = 4096  # H261G_FETCH_INTO_MAX_STREAMS
 OPT_ME_VARIANT = 1
 OPT_MBS_PER_WAVE = 2
 OPT_FUSED_TAILS = 3
@@ -127,6 +128,7 @@ def load() -> ctypes.CDLL:
         "h261g_stream_read_bitstream": (ctypes.c_int, [P, i32, P, i32]),
         "h261g_engine_fetch_bitstreams": (ctypes.c_int, [P, P, i32]),
         "h261g_engine_fetched": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "h261g_engine_fetch_bitstreams_into": (ctypes.c_int, [P, P, i32, P, i64, P]),
         "h261g_stream_read_input": (ctypes.c_int, [P, i32, P, P, P]),
         "h261g_engine_profile": (ctypes.c_int, [P, i32]),
         "h261g_engine_kernel_times": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double),

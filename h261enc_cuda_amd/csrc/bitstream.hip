@@ -21,6 +21,10 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <stdio.h>
+
+#include <mutex>
+
 #include "frame_control.h"
 #include "h261_vlc.h"
 #include "h261gpu_internal.h"
@@ -399,17 +403,20 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void bitstream_kernel(BitLaunc
  * then offsets[n + 1], offsets[n] being the bytes all frames would take. */
 /* one wave: it needs only one wave slot, so it is dispatched at once even
  * while other work fills the device */
-__global__ __launch_bounds__(64) void fetch_plan_kernel(const int32_t *ids, int n, const SeqState *seq,
-                                                        const RcState *rc, int32_t *dev_size, int32_t *dev_off,
-                                                        int32_t *meta, int64_t data_cap)
+__global__ __launch_bounds__(64) void fetch_plan_kernel(const int32_t *ids, int n, const StreamDesc *desc,
+                                                        int n_slots, const SeqState *seq, const RcState *rc,
+                                                        int32_t *dev_size, int32_t *dev_off, int32_t *meta,
+                                                        int64_t data_cap)
 {
     const int lane = threadIdx.x;
     int carry = 0;
     for (int base = 0; base < n; base += 64) {
         const int i = base + lane;
         int size = -1;
-        if (i < n) {
-            const int s = ids[i];
+        /* an id outside the slots or of a slot with no open stream (its
+         * descriptor is zero) has nothing to fetch */
+        const int s = i < n ? ids[i] : -1;
+        if (s >= 0 && s < n_slots && desc[s].mb_w > 0) {
             const SeqState &q = seq[s];
             const RcState &r = rc[s];
             if (q.bits && r.frame_bits > 0 && (r.frame_bits + 31) / 32 <= q.bits_words) size = r.frame_bytes;
@@ -463,12 +470,42 @@ int launch_bitstream(const BitLaunch &p, void *stream, void *ev_start, void *ev_
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_fetch(const int32_t *ids, int n, const SeqState *seq, const RcState *rc, int32_t *dev_size,
-                 int32_t *dev_off, int32_t *meta, uint8_t *host_data, int64_t data_cap, void *stream)
+/* Scratch of the allocation-free fetch: per listed stream its frame size
+ * (first kFetchIntoMaxStreams entries) and its place (next
+ * kFetchIntoMaxStreams).  32 KiB, held by every device the library runs on
+ * until the process ends.  The plan kernel writes every entry the copy kernel
+ * reads, so nothing needs clearing between calls; calls that overlap on one
+ * device share it and race.  Host code never passes the array's name to a
+ * kernel (on the host it is only a stand-in): it passes the device address
+ * from fetch_into_scratch. */
+static __device__ __align__(256) int32_t scratch[2 * kFetchIntoMaxStreams];
+
+int32_t *fetch_into_scratch()
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static int32_t *addr[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) {
+            fprintf(stderr, "h261gpu: hipGetSymbolAddress(scratch): %s\n", hipGetErrorString(err));
+            return nullptr;
+        }
+        addr[dev] = (int32_t *)p;
+    }
+    return addr[dev];
+}
+
+int launch_fetch(const int32_t *ids, int n, const StreamDesc *desc, int n_slots, const SeqState *seq, const RcState *rc,
+                 int32_t *dev_size, int32_t *dev_off, int32_t *meta, uint8_t *host_data, int64_t data_cap, void *stream)
 {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(fetch_plan_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ids, n, seq, rc, dev_size, dev_off,
-                       meta, data_cap);
+    hipLaunchKernelGGL(fetch_plan_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ids, n, desc, n_slots, seq, rc,
+                       dev_size, dev_off, meta, data_cap);
     if (hipGetLastError() != hipSuccess) return -1;
     hipLaunchKernelGGL(fetch_copy_kernel, dim3(n), dim3(256), 0, (hipStream_t)stream, ids, seq, dev_size, dev_off,
                        (uint32_t *)host_data);

@@ -213,6 +213,9 @@ struct h261g_engine {
     int fetch_n = 0;
     bool fetch_pending = false;
     hipEvent_t fetch_ev = nullptr;
+    /* device address of the allocation-free fetch's scratch on the engine's
+     * device (looked up at creation) */
+    int32_t *fetch_scratch = nullptr;
     int mbstruct_cap = 0;
     /* graph of one step */
     /* graphs of one step and of kGraphSteps steps: kernels inside a graph
@@ -719,7 +722,8 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipHostMalloc((void **)&e->remap_host, sizeof(int32_t) * max_streams, hipHostMallocDefault),
                    "hipHostMalloc") &&
                 ok(hipMalloc((void **)&e->remap_dev, sizeof(int32_t) * max_streams), "hipMalloc") &&
-                ok(hipEventCreateWithFlags(&e->remap_ev, hipEventDisableTiming), "event");
+                ok(hipEventCreateWithFlags(&e->remap_ev, hipEventDisableTiming), "event") &&
+                (e->fetch_scratch = fetch_into_scratch()) != nullptr;
     e->remap_cap = good ? (size_t)max_streams : 0;
     for (int r = 0; good && r < h261g_engine::kEvRing; r++)
         for (int i = 0; good && i < 8; i++) good = ok(hipEventCreate(&e->ev[r][i]), "event");
@@ -1351,12 +1355,52 @@ int h261g_engine_fetch_bitstreams(h261g_engine_t *e, const int32_t *ids, int32_t
     int32_t *dev_ids = e->fetch_dev + 2 * e->fetch_cap;
     if (n && (!ok(hipMemcpyAsync(dev_ids, e->fetch_ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream),
                   "hipMemcpy") ||
-              launch_fetch(dev_ids, n, e->seq, e->rc, e->fetch_dev, e->fetch_dev + e->fetch_cap, e->fetch_meta,
-                           e->fetch_data, (int64_t)e->fetch_data_cap, e->stream)))
+              launch_fetch(dev_ids, n, e->desc, e->max_streams, e->seq, e->rc, e->fetch_dev, e->fetch_dev + e->fetch_cap,
+                           e->fetch_meta, e->fetch_data, (int64_t)e->fetch_data_cap, e->stream)))
         return H261G_ERR_DEVICE;
     if (!ok(hipEventRecord(e->fetch_ev, e->stream), "event")) return H261G_ERR_DEVICE;
     e->fetch_pending = true;
     return H261G_OK;
+}
+
+/* memory the device can access in place: device memory, or host memory
+ * allocated pinned (hipHostMalloc) */
+static bool device_accessible(const void *p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeHost || a.type == hipMemoryTypeManaged;
+}
+
+int h261g_engine_fetch_bitstreams_into(h261g_engine_t *e, const int32_t *ids, int32_t n, uint8_t *out, int64_t cap,
+                                       int32_t *meta)
+{
+    if (!e || n < 0 || cap < 0) return H261G_ERR_ARG;
+    if (n > kFetchIntoMaxStreams) {
+        fprintf(stderr, "h261gpu: h261g_engine_fetch_bitstreams_into: %d streams; at most %d per call\n", n,
+                kFetchIntoMaxStreams);
+        return H261G_ERR_ARG;
+    }
+    if (n == 0) return H261G_OK;
+    if (!ids || !out || !meta || ((uintptr_t)out & 3) || ((uintptr_t)meta & 3) || ((uintptr_t)ids & 3)) {
+        fprintf(stderr, "h261gpu: h261g_engine_fetch_bitstreams_into: null or unaligned buffer\n");
+        return H261G_ERR_ARG;
+    }
+    if (!device_accessible(ids) || !device_accessible(out) || !device_accessible(meta)) {
+        fprintf(stderr, "h261gpu: h261g_engine_fetch_bitstreams_into: ids, out and meta must be device memory or "
+                        "pinned host memory\n");
+        return H261G_ERR_ARG;
+    }
+    if (!e->fetch_scratch) return H261G_ERR_STATE;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
+    /* sizes and places in the scratch's two halves */
+    return launch_fetch(ids, n, e->desc, e->max_streams, e->seq, e->rc, e->fetch_scratch,
+                        e->fetch_scratch + kFetchIntoMaxStreams, meta, out, cap, e->stream)
+               ? H261G_ERR_DEVICE
+               : H261G_OK;
 }
 
 int h261g_engine_fetched(h261g_engine_t *e, const uint8_t **data, const int32_t **sizes, const int32_t **offsets)

@@ -237,12 +237,21 @@ struct BitLaunch {
 int launch_bitstream(const BitLaunch &p, void *stream, void *ev_start = nullptr, void *ev_stop = nullptr);
 
 /* the last frame's bitstream of streams ids[0..n) (ids readable by the
- * device) into host memory: sizes and 4-byte-aligned places into meta
- * (sizes[n], offsets[n + 1]; size -1: not on the device or past data_cap
- * bytes), the bytes into host_data; dev_size / dev_off: device scratch of n
- * entries each */
-int launch_fetch(const int32_t *ids, int n, const SeqState *seq, const RcState *rc, int32_t *dev_size,
-                 int32_t *dev_off, int32_t *meta, uint8_t *host_data, int64_t data_cap, void *stream);
+ * device) into host_data (host or device memory the device can write):
+ * sizes and 4-byte-aligned places into meta (sizes[n], offsets[n + 1]; size
+ * -1: not on the device, past data_cap bytes, or an id of no open stream
+ * among the n_slots descriptors), the bytes into host_data; dev_size /
+ * dev_off: device scratch of n entries each */
+int launch_fetch(const int32_t *ids, int n, const StreamDesc *desc, int n_slots, const SeqState *seq, const RcState *rc,
+                 int32_t *dev_size, int32_t *dev_off, int32_t *meta, uint8_t *host_data, int64_t data_cap, void *stream);
+
+/* h261g_engine_fetch_bitstreams_into: at most this many streams per call;
+ * its scratch (sizes and places, kFetchIntoMaxStreams entries each) is a
+ * file-scope device array of bitstream.hip.  fetch_into_scratch gives that
+ * array's address on the current device (looked up once per device), or
+ * nullptr when the lookup fails. */
+constexpr int kFetchIntoMaxStreams = 4096;
+int32_t *fetch_into_scratch();
 
 }  // namespace h261g
 

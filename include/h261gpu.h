@@ -235,6 +235,28 @@ H261G_API int h261g_engine_fetch_bitstreams(h261g_engine_t *e, const int32_t *id
  * fetch. */
 H261G_API int h261g_engine_fetched(h261g_engine_t *e, const uint8_t **data, const int32_t **sizes,
                                    const int32_t **offsets);
+/* Allocation-free bulk bitstream delivery.  No counterpart in the
+ * reference: an entry point for callers that cannot allocate device or host
+ * memory (code under HIP graph capture, serving loops that must not call
+ * hipMalloc / hipHostMalloc).  The same as h261g_engine_fetch_bitstreams,
+ * but into buffers the caller owns, with the plan's scratch (per stream its
+ * size and place) in a fixed device array of the library rather than in
+ * memory the engine allocates; it allocates and frees nothing.
+ *   ids   n stream ids, n <= H261G_FETCH_INTO_MAX_STREAMS
+ *   out   cap bytes for the frames, 4-byte aligned
+ *   meta  2n + 1 int32: sizes[n], then offsets[n + 1] (offsets[n]: the bytes
+ *         all frames would take), as h261g_engine_fetched gives them;
+ *         sizes[i] < 0: frame not on the device, past cap, or ids[i] names no
+ *         open stream
+ * ids, out and meta must be device memory or pinned host memory
+ * (H261G_ERR_ARG otherwise, and for n above the maximum).  Queued behind the
+ * work queued so far on the engine stream; returns at once, and
+ * h261g_engine_sync waits for it.  Every call on one device shares the one
+ * scratch array: calls that overlap on a device (two engines, or two threads)
+ * race. */
+#define H261G_FETCH_INTO_MAX_STREAMS 4096
+H261G_API int h261g_engine_fetch_bitstreams_into(h261g_engine_t *e, const int32_t *ids, int32_t n, uint8_t *out,
+                                                 int64_t cap, int32_t *meta);
 /* the current input planes, for checking device-generated content
  * (synchronises the queued input generation and frame steps) */
 H261G_API int h261g_stream_read_input(h261g_engine_t *e, int32_t id, uint8_t *y, uint8_t *cb, uint8_t *cr);
