@@ -48,10 +48,12 @@ def _p(a: np.ndarray) -> int:
 class Oracle:
     """The C restatement of the reference path (h261_oracle.c)."""
 
-    def __init__(self):
-        if not os.path.exists(ORACLE_SO):
-            raise RuntimeError(f"{ORACLE_SO} missing: run make -C oracle")
-        lib = ctypes.CDLL(ORACLE_SO)
+    def __init__(self, path: str = ORACLE_SO):
+        """path: the oracle library (another build of h261_oracle.c, e.g. a
+        deliberately mutated one in a test, may be given)"""
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run make -C oracle")
+        lib = ctypes.CDLL(path)
         self.lib = lib
         sig = {
             "ora_mv_limits": (None, [I, I, P]),
@@ -129,6 +131,26 @@ class Oracle:
         md = np.zeros(len(me), MODE_DTYPE)
         self.lib.ora_mode_decision(_p(me), len(me), frame_type, _p(md))
         return md
+
+    def encode_macroblocks(self, cur, ref, md: np.ndarray, pic_quant: int, dn: np.ndarray):
+        """The MB coder of one frame (ora_encode_macroblocks) for given mode
+        decisions and denoise state.  cur, ref: (Y, Cb, Cr).  dn: one
+        DENOISE_DTYPE record, updated in place (the next frame's offsets).
+        Returns (macroblocks, reconstruction planes)."""
+        h, w = cur[0].shape
+        n_mb = (w // 16) * (h // 16)
+        assert len(md) == n_mb and dn.dtype == DENOISE_DTYPE and dn.shape == (1,)
+        cur = [np.ascontiguousarray(p, dtype=np.uint8) for p in cur]
+        ref = [np.ascontiguousarray(p, dtype=np.uint8) for p in ref]
+        rec = [np.zeros_like(p) for p in cur]
+        mbs = np.zeros(n_mb, MACROBLOCK_DTYPE)
+        cp = (ctypes.c_void_p * 3)(*[_p(p) for p in cur])
+        rp = (ctypes.c_void_p * 3)(*[_p(p) for p in ref])
+        op = (ctypes.c_void_p * 3)(*[_p(p) for p in rec])
+        m = np.ascontiguousarray(md, dtype=MODE_DTYPE)
+        self.lib.ora_encode_macroblocks(ctypes.addressof(cp), ctypes.addressof(rp), w, h, _p(m), pic_quant, _p(dn),
+                                        _p(mbs), ctypes.addressof(op))
+        return mbs, tuple(rec)
 
     def write_picture(self, mbs: np.ndarray, mb_w: int, source_format: int, tr: int, quant: int,
                       cap: int = 1 << 20) -> bytes:
