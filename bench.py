@@ -85,14 +85,31 @@ def coarse_valid_candidates(mb_w, mb_h):
 
 
 # byte-SADs per inter frame: valid coarse candidates + at most 49 refine
-# candidates per MB, 256 byte-SADs each (an upper bound for the refine part)
+# candidates per MB, 256 byte-SADs each (an upper bound for the refine part;
+# used only when the exact count below is missing)
 SAD_OPS = {396: (coarse_valid_candidates(22, 18) + 49 * 396) * 256,
            99: (coarse_valid_candidates(11, 9) + 49 * 99) * 256}
 
+# the exact candidates the reference's two-stage search evaluates in every
+# frame step of the batched job (valid coarse + valid refine candidates,
+# counted by the oracle over every stream; tests/golden/make_sad_counts.py)
+SAD_COUNTS = os.path.join(ROOT, "tests", "golden", "batched_512_sad_candidates.json")
 
-def algorithmic(specs, t0, t1):
+
+def exact_sad_per_step():
+    """byte-SADs of each frame step of the batched job, or None"""
+    try:
+        with open(SAD_COUNTS) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    return [256 * c for c in d["candidates_per_step"]]
+
+
+def algorithmic(specs, t0, t1, sad_steps=None):
     """algorithmic HBM bytes of the ME and MB kernels and byte-SADs of frame
-    steps t0..t1-1 (intra frames: no reference read, no search)"""
+    steps t0..t1-1 (intra frames: no reference read, no search); byte-SADs
+    from sad_steps (exact, per step) when given, else the upper bound"""
     me = mb = sad = mbs = 0
     for s in specs:
         w, h = s["size"]
@@ -104,6 +121,8 @@ def algorithmic(specs, t0, t1):
         mb += inter * MB_BYTES[nmb] + intra * (2 * fsz + nmb * (24 + 384 + 8))
         sad += inter * SAD_OPS[nmb]
         mbs += n * nmb
+    if sad_steps is not None:
+        sad = sum(sad_steps[t0:t1])
     return me, mb, sad, mbs
 
 
@@ -182,30 +201,61 @@ def serving(rate=2000.0, duration=0.5):
             "max_concurrent_streams": st.max_concurrent, "steps": st.steps}
 
 
-def cpu_baseline(frames=150, workers=None):
-    """The oracle (C restatement of the reference path) on the host cores:
-    one CIF stream of the plain configuration per worker process."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(frames=30, runs=3, max_workers=16):
+    """The oracle (the C restatement of the reference path) on the host
+    cores, in the plain CPU configuration: one CIF stream of `frames` frames
+    (fixed QUANT 8, global motion +-7, noise sigma 2) per worker process,
+    each pinned to its own core of this process's affinity set (at most
+    max_workers: a GPU box's CPU share), distinct seeds, all started together
+    behind a barrier after generating their input.  Median of `runs` runs of
+    aggregate frames/s (total frames / span from the first start to the last
+    end) and of per-core frames/s."""
     import multiprocessing as mp
-    workers = workers or min(16, os.cpu_count() or 1)
-    with mp.get_context("fork").Pool(workers) as pool:
-        t0 = time.perf_counter()
-        res = pool.starmap(_cpu_stream, [(i + 1, frames) for i in range(workers)])
-        dt = time.perf_counter() - t0
-    total = sum(r[0] for r in res)
-    cpu_s = sum(r[1] for r in res)
-    return total / dt, workers, dt, cpu_s
+    cores = sorted(os.sched_getaffinity(0))[:max_workers]
+    ctx = mp.get_context("spawn")  # fresh interpreters: nothing of this process's GPU state
+    agg, per_core, cpu_s = [], [], 0.0
+    for r in range(runs):
+        barrier = ctx.Barrier(len(cores))
+        q = ctx.Queue()
+        procs = [ctx.Process(target=_cpu_stream, args=(c, 1000 * r + i + 1, frames, barrier, q))
+                 for i, c in enumerate(cores)]
+        for p in procs:
+            p.start()
+        res = [q.get(timeout=600) for _ in procs]
+        for p in procs:
+            p.join()
+        t0 = min(x[1] for x in res)
+        t1 = max(x[2] for x in res)
+        agg.append(sum(x[0] for x in res) / (t1 - t0))
+        per_core.append(float(np.median([x[0] / (x[2] - x[1]) for x in res])))
+        cpu_s += sum(x[3] for x in res)
+    return {"value": round(float(np.median(agg)), 2), "per_core": round(float(np.median(per_core)), 2),
+            "cores": len(cores), "runs": [round(a, 2) for a in agg], "cpu_s": round(cpu_s, 1)}
 
 
-def _cpu_stream(seed, frames):
+def _cpu_stream(core, seed, frames, barrier, q):
+    os.sched_setaffinity(0, {core})
     from h261enc_cuda_amd import content_frame
     from oracle.binding import Oracle
-    o = Oracle()
-    s = o.stream(352, 288, fixed_quant=8)
+    s = Oracle().stream(352, 288, fixed_quant=8)
     inputs = [content_frame(seed, 352, 288, t, sigma=2, motion=7) for t in range(frames)]
-    t0 = time.process_time()
+    barrier.wait()
+    c0, t0 = time.process_time(), time.perf_counter()
     for f in inputs:
         s.encode(*f)
-    return frames, time.process_time() - t0
+    t1 = time.perf_counter()
+    q.put((frames, t0, t1, time.process_time() - c0))
 
 
 def main():
@@ -262,6 +312,10 @@ def main():
     eng.destroy()
     cif_equiv = mbs / 396.0
     value = cif_equiv / dt
+    shortest = min(s["length"] for s in specs)
+    timed_window = (f"frame steps {W}..{W + K - 1} of the job's {job}"
+                    + (f" (all {len(specs)} streams active: the shortest has {shortest} frames)"
+                       if W + K <= shortest else " (streams finish and are re-packed during it)"))
 
     # pass B: the same W + K steps again, each kernel bracketed by HIP events
     # on the engine stream; the averages are over the same launches as pass A.
@@ -303,24 +357,40 @@ def main():
     eng.destroy()
     levels_fps = mbs_l / 396.0 / dt_l
 
+    sad_steps = exact_sad_per_step()
+
     def kernel_figures(t, t0, t1):
         n = t["steps"]
-        me_b, mb_b, sad, mbs = algorithmic(specs, t0, t1)
+        me_b, mb_b, sad, mbs = algorithmic(specs, t0, t1, sad_steps)
         me_ms, mb_ms = t["me_ms"] / n, t["mb_ms"] / n
         return dict(launches=n, me_ms=me_ms, mb_ms=mb_ms, me_bytes=me_b / n, mb_bytes=mb_b / n, mbs=mbs / n,
                     me_gbs=me_b / n / (me_ms * 1e-3) / 1e9, mb_gbs=mb_b / n / (mb_ms * 1e-3) / 1e9,
-                    sad_rate=sad / n / (me_ms * 1e-3), other_ms=t["other_ms"] / n, bits_ms=t["bits_ms"] / n)
+                    sad=sad / n, sad_rate=sad / n / (me_ms * 1e-3), other_ms=t["other_ms"] / n,
+                    bits_ms=t["bits_ms"] / n)
 
     job_k = kernel_figures(prof, W, W + K)
     full_k = kernel_figures(first, W, first["done"])
     dominant = "mb_kernel" if job_k["mb_ms"] >= job_k["me_ms"] else "me_kernel"
     key = "mb" if dominant == "mb_kernel" else "me"
-    achieved = job_k[key + "_gbs"]
-    roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_per_launch(dominant, job_k["mbs"], tuned["xcd_remap"]),
-            "avg_launch_ms": round(job_k[key + "_ms"], 4),
-            "algorithmic_bytes_per_launch": int(job_k[key + "_bytes"]), "launches": job_k["launches"]}
+    hbm = {"achieved": round(job_k[key + "_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(job_k[key + "_gbs"] / HBM_PEAK_GBS, 4)}
+    # the binding roof: T_roof = max(bytes / HBM bandwidth, byte-SADs / v_sad_u8
+    # rate) for motion estimation, bytes / HBM bandwidth for the coder
+    t_hbm = job_k[key + "_bytes"] / (HBM_PEAK_GBS * 1e9)
+    t_sad = job_k["sad"] / SAD_PEAK if dominant == "me_kernel" else 0.0
+    if t_sad > t_hbm:
+        achieved = job_k["sad_rate"] / 1e12
+        roof = {"kernel": dominant, "bound": "sad", "achieved": round(achieved, 3), "peak": SAD_PEAK / 1e12,
+                "unit": "T byte-SAD/s", "frac": round(achieved * 1e12 / SAD_PEAK, 4)}
+    else:
+        roof = {"kernel": dominant, "bound": "hbm", **hbm}
+    roof.update({"traffic": traffic_per_launch(dominant, job_k["mbs"], tuned["xcd_remap"]),
+                 "avg_launch_ms": round(job_k[key + "_ms"], 4),
+                 "algorithmic_bytes_per_launch": int(job_k[key + "_bytes"]),
+                 "algorithmic_byte_sads_per_launch": int(job_k["sad"]) if dominant == "me_kernel" else None,
+                 "sad_count": ("exact: the reference's valid candidates per step (tests/golden/"
+                               "batched_512_sad_candidates.json)" if sad_steps else "upper bound (49 refine per MB)"),
+                 "t_roof_ms": round(max(t_hbm, t_sad) * 1e3, 4), "hbm": hbm, "launches": job_k["launches"]})
     for name, f in (("job_average", job_k), ("full_load", full_k)):
         roof[name] = {"launches": f["launches"],
                       "mbs_per_launch": round(f["mbs"], 1),
@@ -346,10 +416,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cfps, cores, cdt, cpu_s = cpu_baseline()
-        cpu = {"value": round(cfps, 2), "unit": "CIF frames/s", "cores": cores, "kind": "port",
-               "sample": f"{cores} CIF streams x 150 frames (fixed QUANT 8, the plain configuration's content), "
-                         f"one per process, {cdt:.1f} s wall, {cpu_s:.1f} s CPU"}
+        c = cpu_baseline()
+        cpu = {"value": c["value"], "unit": "CIF frames/s", "cores": c["cores"], "kind": "port",
+               "per_core": c["per_core"], "cpu_model": cpu_model(), "runs": c["runs"],
+               "sample": f"the plain CPU configuration: one CIF stream of 30 frames (fixed QUANT 8, global motion "
+                         f"+-7, sigma 2, distinct seeds) per process, each pinned to its own core, {c['cores']} "
+                         f"cores; median of 3 runs ({c['cpu_s']} s of CPU in all)"}
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "CIF frames/s", "n_gpus": 1, "steps": K,
@@ -357,7 +429,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": "batched: 256 CIF + 256 QCIF streams, lengths U{60..600}, global motion +-15 px, "
                                "rate control 64-384 kbit/s, inputs resident in HBM, every frame's H.261 bitstream written in HBM "
-                               "by the step; timed: the job's frame steps after the warmup to its end",
+                               f"by the step; timed: {timed_window}",
                    "frames": frames, "cif_equivalent_frames": round(cif_equiv, 1),
                    "frames_per_s": round(frames / dt, 1), "seconds": round(dt, 3),
                    "levels_ready_cif_frames_per_s": round(levels_fps, 1),

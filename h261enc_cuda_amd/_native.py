@@ -85,7 +85,7 @@ class DeviceInfo(ctypes.Structure):
 
 class FrameInfo(ctypes.Structure):
     _fields_ = [("frame_num", ctypes.c_int32), ("frame_type", ctypes.c_int32), ("quant", ctypes.c_int32),
-                ("bytes", ctypes.c_int32)]
+                ("bytes", ctypes.c_int32), ("cost", ctypes.c_int32)]
 
 
 _lib = None

@@ -7,7 +7,8 @@
 
 encode reads I420 frames (Y, Cb, Cr planes back to back, as h261_main.c:
 178-184), codes them on the GPU, writes the bitstream and the
-reconstruction, and prints per-frame quantiser, size, PSNR and time and the
+reconstruction, and prints per-frame quantiser, picture cost, size, PSNR and
+time (h261_main.c:257-264) and the
 summary line (mean PSNR, kbit/s at 29.97 frames/s, frames/s) of
 h261_main.c:257-279.  Rate control defaults to the reference's settings
 (initial quantiser 10, 512 kbit/s, VBV 512 kbit, h261_main.c:165); --quant
@@ -63,8 +64,8 @@ def encode(args) -> int:
             ps = [psnr(a, b) for a, b in zip(f, rec)]
             psnr_sum += ps
             total_bytes += len(bits)
-            print(f"frame {n:04d}, quant={info.quant}, size={len(bits)}, ypsnr={ps[0]:f}, cbpsnr={ps[1]:f}, "
-                  f"crpsnr={ps[2]:f}, time={dt:.3f}ms")
+            print(f"frame {n:04d}, quant={info.quant}, cost={info.cost}, size={len(bits)}, ypsnr={ps[0]:f}, "
+                  f"cbpsnr={ps[1]:f}, crpsnr={ps[2]:f}, time={dt:.3f}ms")
             n += 1
     elapsed = time.perf_counter() - t_start
     eng.destroy()

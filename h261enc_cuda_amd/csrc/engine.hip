@@ -1199,6 +1199,7 @@ int h261g_stream_info(h261g_engine_t *e, int32_t id, h261g_frame_info_t *info)
     info->frame_type = d.frame_num - 1 == 0 ? H261_FRAME_TYPE_INTRA : H261_FRAME_TYPE_INTER;
     info->quant = rc.pic_quant;
     info->bytes = rc.frame_bytes;
+    info->cost = rc.picture_cost;
     return H261G_OK;
 }
 

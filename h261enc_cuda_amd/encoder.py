@@ -9,6 +9,7 @@ write_picture the H.261 writer over a frame's macroblock structs
 from __future__ import annotations
 
 import ctypes
+import math
 from typing import List, Optional, Tuple
 
 import numpy as np
@@ -301,5 +302,10 @@ class LegacyDevice:
 
 
 def psnr(a: np.ndarray, b: np.ndarray) -> float:
-    d = np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)
-    return 100.0 if d == 0 else 10 * np.log10(255.0 ** 2 / d)
+    """PSNR of one plane as src/h261_main.c:38-91 computes it: the integer
+    sum of squared differences, 100.0 when it is zero, else
+    10 log10(255.0 * 255.0 * pels / ssd) in double, in that order"""
+    ssd = int(((a.astype(np.int64) - b.astype(np.int64)) ** 2).sum())
+    if ssd == 0:
+        return 100.0
+    return 10 * math.log10(255.0 * 255.0 * (a.shape[0] * a.shape[1]) / float(ssd))

@@ -206,6 +206,8 @@ typedef struct {
     int32_t frame_type; /* H261_FRAME_TYPE_* */
     int32_t quant;      /* picture quantiser */
     int32_t bytes;      /* exact H.261 size of the frame in bytes */
+    int32_t cost;       /* picture cost rate control used: the sum of the MBs' mode-decision
+                           costs, at least 100 per MB (h261_ratectrl.c:225-245) */
 } h261g_frame_info_t;
 H261G_API int h261g_stream_info(h261g_engine_t *e, int32_t id, h261g_frame_info_t *info);
 /* macroblocks of the last frame in the reference's h261_macroblock_t

@@ -70,6 +70,7 @@ class Oracle:
             "ora_stream_destroy": (None, [P]),
             "ora_stream_encode_frame": (I, [P, P, P, P, P, I, P, P, P, P, P, P]),
             "ora_stream_last_sad_candidates": (ctypes.c_long, [P]),
+            "ora_stream_last_picture_cost": (I, [P]),
             "ora_mode_decision": (None, [P, I, I, P]),
             "ora_rc_init": (None, [P, I, I, I, I]),
             "ora_rc_picture_cost": (I, [P, I]),
@@ -199,6 +200,9 @@ class OracleStream:
 
     def sad_candidates(self) -> int:
         return self.o.lib.ora_stream_last_sad_candidates(self.handle)
+
+    def picture_cost(self) -> int:
+        return self.o.lib.ora_stream_last_picture_cost(self.handle)
 
     def close(self):
         if self.handle:

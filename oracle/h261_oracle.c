@@ -963,6 +963,10 @@ void ora_stream_destroy(ora_stream_t *s)
 
 long ora_stream_last_sad_candidates(const ora_stream_t *s) { return s->sad_candidates; }
 
+/* the last frame's picture cost as rate control used it
+ * (s_ratectrl.i_current_picture_cost, printed by h261_main.c:257-264) */
+int ora_stream_last_picture_cost(const ora_stream_t *s) { return s->rc.i_current_picture_cost; }
+
 int ora_stream_encode_frame(ora_stream_t *s, const uint8_t *y, const uint8_t *cb, const uint8_t *cr,
                             uint8_t *bits, int cap, uint8_t *recon_y, uint8_t *recon_cb, uint8_t *recon_cr,
                             me_gpu_mb_t *me_out, h261_macroblock_t *mb_out, int *quant_out)

@@ -82,6 +82,7 @@ int ora_stream_encode_frame(ora_stream_t *s, const uint8_t *y, const uint8_t *cb
 /* Number of SAD candidates the two-stage search evaluated in the last frame
  * (valid coarse + valid refine); used to price the SAD roofline. */
 long ora_stream_last_sad_candidates(const ora_stream_t *s);
+int ora_stream_last_picture_cost(const ora_stream_t *s);
 
 /* ---- pieces of the frame loop, exposed for tests ---- */
 
