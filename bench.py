@@ -272,7 +272,8 @@ def main():
     ap.add_argument("--compact-every", type=int, default=30,
                     help="steps between closing finished streams and re-packing the MB space")
     ap.add_argument("--shape", default=None,
-                    help="MBS_PER_WAVE,XCD_REMAP: a fixed launch shape instead of the autotuner's (profiling)")
+                    help="MBS_PER_WAVE,XCD_REMAP[,ME_MBS_PER_WAVE]: a fixed launch shape instead of the "
+                         "autotuner's (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -295,8 +296,9 @@ def main():
     # workload before anything is timed
     from h261enc_cuda_amd import autotune
     if args.shape:
-        mpw, remap = (int(v) for v in args.shape.split(","))
-        tuned = {"mbs_per_wave": mpw, "xcd_remap": remap, "fixed": True}
+        v = [int(x) for x in args.shape.split(",")]
+        tuned = {"mbs_per_wave": v[0], "xcd_remap": v[1], "me_mbs_per_wave": v[2] if len(v) > 2 else 0,
+                 "fixed": True}
     else:
         tuned = autotune.tune([s["size"] for s in specs])
 

@@ -65,6 +65,7 @@ OPT_ASYNC_INPUT = 5
 OPT_DEVICE_BITSTREAM = 6
 OPT_OVERLAP_WRITER = 7
 OPT_CHAINS = 8
+OPT_ME_MBS_PER_WAVE = 9
 
 
 class StreamParams(ctypes.Structure):

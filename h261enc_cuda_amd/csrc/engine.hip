@@ -170,6 +170,7 @@ struct h261g_engine {
     int chains = 0; /* 0 by size, 1 one chain, 2 two chains */
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
+    int me_mbs_per_wave = 0; /* motion estimation: 0 chosen per launch, else 1, 2, 4 or 8 */
     int me_variant = kMeTwoStage;
     /* 1: rate-control begin in the ME tail; 2: its own launch; 3: its own
      * launch when a chain fills the device (a short step is latency bound),
@@ -420,6 +421,10 @@ int read_oldest_times(h261g_engine *e);
 /* waves of the macroblock coder that keep every SIMD of the device busy
  * (256 CUs x 4 SIMDs x 5 waves) */
 constexpr int kFillWaves = 256 * 4 * 5;
+/* motion-estimation waves resident on the device (256 CUs x 4 SIMDs x 7) and
+ * the MBs per wave of a launch that holds several times as many MBs */
+constexpr int kMeResidentWaves = 256 * 4 * 7;
+constexpr int kMeRunDefault = 2;
 
 /* one frame step: three launches (default), or the six standalone launches */
 /* the bitstream writer over the whole MB space (MBs not placed by a frame
@@ -467,6 +472,15 @@ int chain_split(const h261g_engine *e, bool timed)
     return best;
 }
 
+/* MBs one motion-estimation wave searches in turn: the option, else by
+ * launch size -- one while a launch is short (latency), more once the
+ * launch holds several times the waves the device keeps resident */
+int me_mbs_per_wave(const h261g_engine *e, int n_mb)
+{
+    if (e->me_mbs_per_wave) return e->me_mbs_per_wave;
+    return n_mb >= 4 * kMeResidentWaves ? kMeRunDefault : 1;
+}
+
 /* one frame step of the streams whose MBs lie in [lo, hi), on stream st */
 int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, bool timed, hipEvent_t *ev,
                   hipEvent_t me_done = nullptr)
@@ -488,6 +502,7 @@ int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, b
     ml.variant = e->me_variant;
     ml.fuse_tail = fused == 1;
     ml.xcd_remap = e->xcd_remap & 1;
+    ml.mbs_per_wave = me_mbs_per_wave(e, hi - lo);
     /* one chain: the previous step's bitstreams beside this step's motion
      * estimation (with nothing pending every MB is skipped) */
     const bool overlap =
@@ -655,6 +670,10 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
     case H261G_OPT_CHAINS:
         if (value < 0 || value > 2) return H261G_ERR_ARG;
         e->chains = value;
+        break;
+    case H261G_OPT_ME_MBS_PER_WAVE:
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return H261G_ERR_ARG;
+        e->me_mbs_per_wave = value;
         break;
     case H261G_OPT_DEVICE_BITSTREAM:
         if (value < 0 || value > 2) return H261G_ERR_ARG;

@@ -173,6 +173,7 @@ struct MeLaunch {
     int32_t fuse_tail;            /* 1: the MB that completes a stream runs rate-control begin;
                                      0: costs only accumulate (rate-control begin is launched) */
     int32_t xcd_remap;            /* 1: contiguous macroblock runs per XCD (xcd_logical) */
+    int32_t mbs_per_wave;         /* consecutive MBs one wave searches in turn (1, 2, 4 or 8; 0 = 1) */
 };
 enum : int32_t {
     kMeTwoStage = 0, /* the reference's candidates only, one v_sad_u8 lane per candidate */

@@ -230,28 +230,6 @@ static_assert(refine_rows_ok(), "refine lane layout");
 __constant__ CoarseLanes c_coarse = make_coarse_lanes();
 __constant__ RefineLanes c_refine = make_refine_lanes();
 
-/* lane masks of the in-limit candidates (wave-uniform, so scalar work).
- * The valid coarse candidates are a product of a tx range and a ty range
- * (coarse_vec is increasing), so slot i's mask is the lanes with ty in
- * [y0, y1] and tx in [x0, x1]. */
-__device__ __forceinline__ void coarse_masks(int lx0, int lx1, int ly0, int ly1, uint64_t &m0, uint64_t &m1)
-{
-    uint32_t xm = 0, ym = 0;
-#pragma unroll
-    for (int t = 0; t < 11; t++) {
-        const int v = coarse_vec(t);
-        xm |= (uint32_t)(v >= lx0 && v <= lx1) << t;
-        ym |= (uint32_t)(v >= ly0 && v <= ly1) << t;
-    }
-    /* (0, 0) is always in the limits, so neither range is empty */
-    const int x0 = __builtin_ctz(xm), x1 = 31 - __builtin_clz(xm);
-    const int y0 = __builtin_ctz(ym), y1 = 31 - __builtin_clz(ym);
-    m0 = (c_coarse.row_below[0][y1 + 1] & ~c_coarse.row_below[0][y0]) &
-         (c_coarse.col_below[0][x1 + 1] & ~c_coarse.col_below[0][x0]);
-    m1 = (c_coarse.row_below[1][y1 + 1] & ~c_coarse.row_below[1][y0]) &
-         (c_coarse.col_below[1][x1 + 1] & ~c_coarse.col_below[1][x0]);
-}
-
 __device__ __forceinline__ uint64_t refine_mask(int sx, int sy, int lx0, int lx1, int ly0, int ly1)
 {
     const int x0 = max(lx0 - sx + 3, 0), x1 = min(lx1 - sx + 3, 6);
@@ -288,9 +266,15 @@ __device__ __forceinline__ void sad16(const uint32_t *win, const uint8_t *cur_mb
 {
 #pragma unroll
     for (int i = 0; i < NS; i++) acc[i] = 0;
+    /* each search stage loads the current rows anew, next to their use: kept
+     * from one stage to the next they would hold 64 SGPRs and spill */
+    const uint64_t cm = (uint64_t)(uintptr_t)cur_mb;
+    uint64_t base = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(cm >> 32)) << 32) |
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)cm);
+    asm volatile("" : "+s"(base));
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const scalar_u32_ptr c = (scalar_u32_ptr)(cur_mb + (size_t)k * pitch);
+        const scalar_u32_ptr c = (scalar_u32_ptr)(uintptr_t)(base + (uint64_t)k * pitch);
         const uint32_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
 #pragma unroll
         for (int i = 0; i < NS; i++) acc[i] = sad_row(win + off[i] + k * kPitch, sh[i], c0, c1, c2, c3, acc[i]);
@@ -379,32 +363,60 @@ __device__ __forceinline__ void sad_surface(const uint32_t *win, const uint32_t 
     out[1] = make_uint4((uint32_t)acc2, (uint32_t)(acc2 >> 32), (uint32_t)acc3, (uint32_t)(acc3 >> 32));
 }
 
-template <int kVariant>
-__global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
+/* Lane masks of the in-limit coarse candidates by frame edge.  An MB's
+ * limits are +-15 clipped at the frame border (gpu_device.cu:226-265) and MB
+ * positions are multiples of 16, so each limit has one of two values: -15
+ * or 0 on the left and top, 15 or 0 on the right and bottom.  Entry
+ * (left | right << 1 | top << 2 | bottom << 3) holds the masks of the two
+ * coarse slots; spare lanes are never valid. */
+struct CoarseEdgeMasks {
+    uint64_t m[16][2];
+};
+constexpr CoarseEdgeMasks make_coarse_edge_masks()
 {
-    constexpr int kPitch = win_pitch<kVariant>();
-    __shared__ __align__(16) uint32_t winmem[win_alloc<kVariant>()];
-    uint32_t *const win = winmem + win_base<kVariant>();
-    __shared__ __align__(16) uint32_t curw[kVariant == kMeSurface ? 64 : 4];
-    __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
+    CoarseEdgeMasks t{};
+    for (int e = 0; e < 16; e++) {
+        const int lx0 = (e & 1) ? 0 : -15, lx1 = (e & 2) ? 0 : 15, ly0 = (e & 4) ? 0 : -15, ly1 = (e & 8) ? 0 : 15;
+        for (int i = 0; i < 2; i++)
+            for (int l = 0; l < 64; l++) {
+                const int c = kCoarseOrder[2 * i + l / 32][l % 32];
+                if (c < 0) continue;
+                const int vx = coarse_vec(c % 11), vy = coarse_vec(c / 11);
+                if (vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1) t.m[e][i] |= 1ull << l;
+            }
+    }
+    return t;
+}
+__constant__ CoarseEdgeMasks c_coarse_edge = make_coarse_edge_masks();
 
-    const int g = p.mb_first + launch_index(blockIdx.x, p.n_mb - p.mb_first, p.xcd_remap);
-    if (g >= p.n_mb) return;
+/* One macroblock of a W-pel-wide frame (352 CIF, 176 QCIF: every size and
+ * divisor below is a compile-time constant, so MB positions, plane offsets
+ * and the current MB's scalar row loads need no run-time arithmetic). */
+template <int kVariant, int W>
+__device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc &d, uint32_t *win, uint32_t *curw,
+                                     uint16_t *surf)
+{
+    constexpr int H = W == 352 ? 288 : 144, MBW = W / 16, MBH = H / 16;
+    constexpr int kPitch = win_pitch<kVariant>();
     const int lane = threadIdx.x;
-    const int s = p.mb_stream[g];
-    if (s < 0) return; /* unmapped range of a closed stream */
-    const StreamDesc &d = p.streams[s];
-    if (!(d.flags & kStreamActive)) return;
     const int m = g - d.mb_base;
-    const int W = d.width, H = d.height;
-    const int bx = (m % d.mb_w) * 16, by = (m / d.mb_w) * 16;
+    const int mx = m % MBW, my = m / MBW;
+    const int bx = 16 * mx, by = 16 * my;
     const uint8_t *cur = d.cur[0];
     const uint8_t *ref = d.ref[0];
-    const uint8_t *cur_mb = cur + (size_t)by * W + bx;
+    const uint8_t *cur_mb = cur + by * W + bx;
+    const int edges = (mx == 0 ? 1 : 0) | (mx == MBW - 1 ? 2 : 0) | (my == 0 ? 4 : 0) | (my == MBH - 1 ? 8 : 0);
 
     /* motion vector limits (gpu_device.cu:226-265) */
-    const int lx0 = max(bx - 15, 0) - bx, lx1 = min(bx + 15, (d.mb_w - 1) * 16) - bx;
-    const int ly0 = max(by - 15, 0) - by, ly1 = min(by + 15, (d.mb_h - 1) * 16) - by;
+    const int lx0 = (edges & 1) ? 0 : -15, lx1 = (edges & 2) ? 0 : 15;
+    const int ly0 = (edges & 4) ? 0 : -15, ly1 = (edges & 8) ? 0 : 15;
+
+    /* SATD lanes: (4x4 sub-block, row y); this lane's 4 current pels, loaded
+     * first so the load overlaps the staging */
+    const int sub = lane >> 2, yq = lane & 3, sub_x = sub & 3, sub_y = sub >> 2;
+    const int R = 4 * sub_y + yq; /* MB row */
+    const uint32_t crow = *(const uint32_t *)(cur_mb + R * W + 4 * sub_x);
+    const uint32_t cw = kVariant == kMeSurface ? *(const uint32_t *)(cur_mb + (lane >> 2) * W + 4 * (lane & 3)) : 0u;
 
     /* stage the window: all loads issued before any wait.  Lane (c4, rr) of
      * lanes 0-59 loads dword c4 of rows rr, rr + 5, ... (row 45 only for
@@ -418,14 +430,14 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
     uint32_t staged[kStageIters];
     /* buffer loads over the luma plane: a 32-bit lane offset, and in the
      * interior case a scalar row offset (addresses stay in the plane, so its
-     * range check never applies) */
-    /* the plane's address and size are wave-uniform: said so explicitly,
-     * else the resource is built per lane and each load loops over lanes */
+     * range check never applies).  The plane's address is wave-uniform: said
+     * so explicitly, else the resource is built per lane and each load loops
+     * over lanes */
     const uint64_t ref_u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uintptr_t)ref >> 32)) << 32) |
                            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uintptr_t)ref);
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(uintptr_t)ref_u, 0, __builtin_amdgcn_readfirstlane(W * H), kBufferDword3);
-    if (by >= 15 && by + 30 < H) {
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)ref_u, 0, W * H, kBufferDword3);
+    if (!(edges & 12)) {
         const uint32_t lo = (uint32_t)(rr * W + fx);
 #pragma unroll
         for (int t = 0; t < kStageIters - 1; t++)
@@ -438,7 +450,6 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
             staged[t] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)(fy * W + fx), 0, 0);
         }
     }
-    const uint32_t cw = *(const uint32_t *)(cur + (size_t)(by + (lane >> 2)) * W + bx + 4 * (lane & 3));
     if (lane < 5 * kWinDw) {
 #pragma unroll
         for (int t = 0; t < kStageIters - 1; t++) win[(rr + 5 * t) * kPitch + c4] = staged[t];
@@ -473,8 +484,6 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
              * evaluated where it is; the mask bit of an out-of-limit one (or
              * of a spare lane) replaces its cost by 65535 */
             static_assert(kVariant == kMeSurface || kPitch == kTwoStagePitch, "lane tables assume the two-stage pitch");
-            uint64_t m[2];
-            coarse_masks(lx0, lx1, ly0, ly1, m[0], m[1]);
             int off[2], sh[2];
             uint32_t e[2];
 #pragma unroll
@@ -487,7 +496,7 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
             sad16<2, kPitch>(win, cur_mb, W, off, sh, acc);
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                const uint32_t cost = __builtin_amdgcn_inverse_ballot_w64(m[i]) ? acc[i] : 65535u;
+                const uint32_t cost = __builtin_amdgcn_inverse_ballot_w64(c_coarse_edge.m[edges][i]) ? acc[i] : 65535u;
                 const uint32_t key = (cost << 8) | (e[i] >> 16);
                 best = key < best ? key : best;
             }
@@ -523,11 +532,8 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
         mvy = sy + wty - 3;
     }
 
-    /* SATD costs (inter, inter + loop filter, intra): lane = (4x4 sub-block,
-     * row y); 4 pels per lane as packed int16 pairs (p0, p2), (p1, p3) */
-    const int sub = lane >> 2, yq = lane & 3, sub_x = sub & 3, sub_y = sub >> 2;
-    const int R = 4 * sub_y + yq; /* MB row */
-    const uint32_t crow = (uint32_t)__shfl((int)cw, 4 * R + sub_x, 64);
+    /* SATD costs (inter, inter + loop filter, intra); 4 pels per lane as
+     * packed int16 pairs (p0, p2), (p1, p3) */
     const short2v ce = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c020c00u));
     const short2v co = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c030c01u));
     /* window: row of the predicted MB row, byte of pel -1 of this lane */
@@ -536,12 +542,14 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
     const int q = b0 >> 2, bsh = b0 & 3, q2 = min(q + 2, kWinDw - 1);
     const int yl = R & 7;
     const bool row_border = yl == 0 || yl == 7;
-    /* [1 2 1] along the row, as packed pairs (h0, h2) and (h1, h3) */
+    /* [1 2 1] along rows wr - 1, wr, wr + 1 as packed pairs (h0, h2) and
+     * (h1, h3).  Rows -1 and 46 are spare window rows; on a block's border
+     * rows the filtered values are discarded below, so every lane reads the
+     * same three rows and no lane branches */
     uint32_t h02[3], h13[3], pm02 = 0, pm13 = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const int r = row_border ? wr : wr - 1 + i;
-        const uint32_t *w = win + r * kPitch;
+        const uint32_t *w = win + (wr - 1 + i) * kPitch;
         const uint32_t w0 = w[q], w1 = w[q + 1], w2 = w[q2];
         const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, bsh); /* pels -1, 0, 1, 2 */
         const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, bsh); /* pels 3, 4 */
@@ -594,28 +602,71 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
                     : k == 2 ? (md_intra ? 0 : mvx) : k == 3 ? (md_intra ? 0 : mvy) : k == 4 ? 0 : md_cost;
         ((int32_t *)(p.mode + g))[k] = w;
     }
-    if (lane == 0) {
-        if (p.mv_start) {
-            p.mv_start[2 * g] = (int8_t)mvx;
-            p.mv_start[2 * g + 1] = (int8_t)mvy;
-        }
-        if (p.mode && p.rc) {
-            if (p.fuse_tail) {
-                /* count and cost sum in one atomic: the MB that completes
-                 * the count holds the frame's cost sum and starts the picture
-                 * (rate control, quantiser) */
-                RcState &rc = p.rc[s];
-                const unsigned long long old =
-                    atomicAdd((unsigned long long *)&rc.me_acc, (1ull << 32) | (uint32_t)md_cost);
-                if ((int)(old >> 32) == d.mb_w * d.mb_h - 1) {
-                    fc::rc_begin_one(rc, d, (int)(uint32_t)old + md_cost);
-                    atomicExch((unsigned long long *)&rc.me_acc, 0ull);
-                }
-            } else {
-                atomicAdd(&p.rc[s].cost_accum, md_cost);
-            }
-        }
+    if (lane == 0 && p.mv_start) {
+        p.mv_start[2 * g] = (int8_t)mvx;
+        p.mv_start[2 * g + 1] = (int8_t)mvy;
     }
+    return md_cost;
+}
+
+/* The MB costs a wave found for one stream enter its rate control with one
+ * atomic.  Fused tail: count and cost sum travel in one 64-bit atomic, and
+ * the wave whose MBs complete the count holds the frame's cost sum and
+ * starts the picture (rate control, quantiser); else the costs only
+ * accumulate for the rate-control begin launch. */
+__device__ __forceinline__ void add_costs(const MeLaunch &p, int s, const StreamDesc &d, int count, uint32_t cost)
+{
+    if (threadIdx.x != 0 || count == 0 || !p.mode || !p.rc) return;
+    RcState &rc = p.rc[s];
+    if (p.fuse_tail) {
+        const unsigned long long old = atomicAdd((unsigned long long *)&rc.me_acc, ((unsigned long long)count << 32) | cost);
+        if ((int)(old >> 32) + count == d.mb_w * d.mb_h) {
+            fc::rc_begin_one(rc, d, (int)((uint32_t)old + cost));
+            atomicExch((unsigned long long *)&rc.me_acc, 0ull);
+        }
+    } else {
+        atomicAdd(&rc.cost_accum, (int)cost);
+    }
+}
+
+/* One wave searches kMbs consecutive MBs of the global MB space in turn (the
+ * launch's MB range in runs of kMbs): the wave's start-up and the tail of
+ * each MB's stores and atomics overlap the next MB's work, and a run's MBs
+ * of one stream add their costs with one atomic. */
+template <int kVariant, int kMbs>
+__global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
+{
+    __shared__ __align__(16) uint32_t winmem[win_alloc<kVariant>()];
+    __shared__ __align__(16) uint32_t curw[kVariant == kMeSurface ? 64 : 4];
+    __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
+    uint32_t *const win = winmem + win_base<kVariant>();
+
+    const int runs = (p.n_mb - p.mb_first + kMbs - 1) / kMbs;
+    const int r = launch_index(blockIdx.x, runs, p.xcd_remap);
+    if (r >= runs) return;
+    const int g0 = p.mb_first + r * kMbs;
+    int seg = -1, count = 0;
+    uint32_t cost = 0;
+    const StreamDesc *seg_d = nullptr;
+#pragma unroll 1
+    for (int i = 0; i < kMbs; i++) {
+        const int g = g0 + i;
+        if (g >= p.n_mb) break;
+        const int s = p.mb_stream[g]; /* < 0: unmapped range of a closed stream */
+        if (s != seg) {
+            if (seg >= 0) add_costs(p, seg, *seg_d, count, cost);
+            seg = s;
+            count = 0;
+            cost = 0;
+            seg_d = s >= 0 ? &p.streams[s] : nullptr;
+        }
+        if (s < 0 || !(seg_d->flags & kStreamActive)) continue;
+        if (count) __syncthreads(); /* the previous MB's window reads are done */
+        cost += (uint32_t)(seg_d->width == 352 ? me_mb<kVariant, 352>(p, g, *seg_d, win, curw, surf)
+                                               : me_mb<kVariant, 176>(p, g, *seg_d, win, curw, surf));
+        count++;
+    }
+    if (seg >= 0) add_costs(p, seg, *seg_d, count, cost);
 }
 
 }  // namespace
@@ -629,8 +680,14 @@ int launch_me(const MeLaunch &p, void *stream, void *ev_start, void *ev_stop)
         }
         return 0;
     }
-    const dim3 grid(launch_grid(p.n_mb - p.mb_first, p.xcd_remap));
-    auto k = p.variant == kMeSurface ? me_kernel<kMeSurface> : me_kernel<kMeTwoStage>;
+    /* the surface variant searches one MB per wave (its LDS surface and
+     * registers leave no room for a run) */
+    const int mbs = p.variant == kMeSurface ? 1
+                    : p.mbs_per_wave == 8 ? 8 : p.mbs_per_wave == 4 ? 4 : p.mbs_per_wave == 2 ? 2 : 1;
+    const dim3 grid(launch_grid((p.n_mb - p.mb_first + mbs - 1) / mbs, p.xcd_remap));
+    auto k = p.variant == kMeSurface ? me_kernel<kMeSurface, 1>
+             : mbs == 8 ? me_kernel<kMeTwoStage, 8> : mbs == 4 ? me_kernel<kMeTwoStage, 4>
+             : mbs == 2 ? me_kernel<kMeTwoStage, 2> : me_kernel<kMeTwoStage, 1>;
     if (ev_start)
         hipExtLaunchKernelGGL(k, grid, dim3(64), 0, (hipStream_t)stream, (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0,
                               p);

@@ -154,6 +154,9 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                               0 (default): 2 when a step has at least twice the macroblocks that
                               fill the device, else 1.  Profiled steps
                               (h261g_engine_profile) always run as one sequence. */
+#define H261G_OPT_ME_MBS_PER_WAVE 9 /* consecutive macroblocks one motion-estimation wave searches in
+                                       turn: 1, 2, 4 or 8; 0 (default): chosen per launch from its
+                                       size (1 while a launch is short) */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */

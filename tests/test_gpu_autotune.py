@@ -10,10 +10,10 @@ from h261enc_cuda_amd.encoder import CIF, QCIF
 pytestmark = pytest.mark.gpu
 
 
-def encode(sizes, mpw, remap, n=4):
+def encode(sizes, mpw, remap, me_mpw=0, n=4):
     eng = Engine(max_streams=len(sizes))
     try:
-        autotune.apply(eng, {"mbs_per_wave": mpw, "xcd_remap": remap})
+        autotune.apply(eng, {"mbs_per_wave": mpw, "xcd_remap": remap, "me_mbs_per_wave": me_mpw})
         sids = []
         for i, size in enumerate(sizes):
             sid = eng.open(*size, bitrate=128000, vbv_size=128000)
@@ -31,12 +31,13 @@ def encode(sizes, mpw, remap, n=4):
 
 def test_autotune_picks_a_measured_candidate_and_keeps_results():
     sizes = [CIF] * 3 + [QCIF] * 3
-    announce("autotune 6 streams, 2 x 2 candidates")
-    r = autotune.tune(sizes, mbs_per_wave=(1, 4), xcd_remap=(0, 2), steps=2)
-    assert r["mbs_per_wave"] in (1, 4) and r["xcd_remap"] in (0, 2)
-    assert len(r["trials"]) == 4 and all(t["step_ms"] > 0 for t in r["trials"])
+    announce("autotune 6 streams, 2 + 2 + 2 candidates")
+    r = autotune.tune(sizes, mbs_per_wave=(1, 4), xcd_remap=(0, 2), me_mbs_per_wave=(1, 2), steps=2)
+    assert r["mbs_per_wave"] in (1, 4) and r["xcd_remap"] in (0, 2) and r["me_mbs_per_wave"] in (1, 2)
+    assert len(r["trials"]) == 6 and all(t["step_ms"] > 0 for t in r["trials"])
+    pick = (r["mbs_per_wave"], r["xcd_remap"], r["me_mbs_per_wave"])
     assert r["step_ms"] == min(t["step_ms"] for t in r["trials"]
-                               if (t["mbs_per_wave"], t["xcd_remap"]) == (r["mbs_per_wave"], r["xcd_remap"]))
-    other = (1 if r["mbs_per_wave"] != 1 else 4, 2 if r["xcd_remap"] != 2 else 0)
-    announce(f"autotune shapes {r['mbs_per_wave']},{r['xcd_remap']} vs {other}")
-    assert encode(sizes, r["mbs_per_wave"], r["xcd_remap"]) == encode(sizes, *other)
+                               if (t["mbs_per_wave"], t["xcd_remap"], t["me_mbs_per_wave"]) == pick)
+    other = (1 if r["mbs_per_wave"] != 1 else 4, 2 if r["xcd_remap"] != 2 else 0, 2 if r["me_mbs_per_wave"] != 2 else 1)
+    announce(f"autotune shapes {pick} vs {other}")
+    assert encode(sizes, *pick) == encode(sizes, *other)

@@ -333,6 +333,42 @@ def test_engine_two_chains_match_one_chain(oracle):
         eng.destroy()
 
 
+@pytest.mark.parametrize("fused", [1, 2], ids=["rc-begin-in-me", "rc-begin-launch"])
+def test_engine_me_runs_agree(oracle, fused):
+    """Motion estimation with 1, 2, 4 or 8 consecutive MBs per wave (runs
+    that cross stream boundaries, one cost atomic per stream segment, and in
+    the fused form the wave completing a stream's count starting its
+    picture) codes the same bytes as the oracle."""
+    from h261enc_cuda_amd._native import OPT_FUSED_TAILS, OPT_ME_MBS_PER_WAVE
+    specs = [(QCIF, 311), (CIF, 312), (QCIF, 313), (CIF, 314), (QCIF, 315)]
+    n = 4
+    results = []
+    for runs in (1, 2, 4, 8):
+        announce(f"motion estimation {runs} MBs per wave, fused={fused}")
+        eng = Engine(max_streams=len(specs))
+        eng.config(OPT_FUSED_TAILS, fused)
+        eng.config(OPT_ME_MBS_PER_WAVE, runs)
+        ids = []
+        for size, seed in specs:
+            sid = eng.open(*size, bitrate=96000, vbv_size=96000)
+            eng.generate_input(sid, seed, n, sigma=3, motion=15)
+            ids.append(sid)
+        out = []
+        for _ in range(n):
+            eng.step()
+            eng.sync()
+            out.append([(eng.bitstream(sid), eng.info(sid).quant) for sid in ids])
+        eng.destroy()
+        results.append(out)
+    for r in results[1:]:
+        assert r == results[0]
+    for k, (size, seed) in enumerate(specs):
+        s = oracle.stream(*size, bitrate=96000, vbv=96000)
+        for t in range(n):
+            ob, _, _, _, oq = s.encode(*content_frame(seed, *size, t, sigma=3, motion=15))
+            assert results[0][t][k] == (ob, oq), f"stream {k} frame {t}"
+
+
 def test_engine_launch_shapes_agree(oracle):
     """Fused and unfused frame tails, rate-control begin in the ME tail or as
     its own launch, 1/3/8 MBs per wave (workgroups that span several
