@@ -93,6 +93,33 @@ constexpr int kBlkPitch = 72; /* floats per block in the transpose buffer */
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+/* The plane pointers come from stream descriptors in memory, so the compiler
+ * sees generic pointers and would issue flat loads and stores.  A flat
+ * access also counts against the LDS wait counter: every wait for an LDS
+ * result would then wait for the outstanding plane accesses as well.  The
+ * plane accesses name the global address space instead. */
+template <class T>
+__device__ __forceinline__ T load_global(const uint8_t *p)
+{
+    return *(const __attribute__((address_space(1))) T *)(p);
+}
+/* two words through a plain vector type: uint2's member operators take
+ * their operands as generic pointers and would bring the flat access back */
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void load_global2(const uint8_t *p, uint32_t &w0, uint32_t &w1)
+{
+    const u32x2 v = *(const __attribute__((address_space(1))) u32x2 *)(p);
+    w0 = v.x;
+    w1 = v.y;
+}
+__device__ __forceinline__ void store_global2(uint8_t *p, uint32_t w0, uint32_t w1)
+{
+    u32x2 v;
+    v.x = w0;
+    v.y = w1;
+    *(__attribute__((address_space(1))) u32x2 *)(p) = v;
+}
+
 /* the 1-D AAN forward transform shared by the row and column passes
  * (mbgpu:350-387 and 399-437); o[k] gets output k before the postscale */
 struct Fwd8 {
@@ -351,7 +378,8 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             uint32_t predw[2] = {0, 0};
             if (L.act) {
                 const MbInfo &mbi = info[L.mi];
-                const uint2 cw = *(const uint2 *)(mbi.cur[L.plane] + (size_t)(L.py + rc) * L.stride + L.px);
+                uint32_t cw0, cw1;
+                load_global2(mbi.cur[L.plane] + (size_t)(L.py + rc) * L.stride + L.px, cw0, cw1);
                 const int rx = L.px + L.mvx;
                 const int a = rx & ~3, o = rx & 3;
                 /* the third dword is needed only when o > 0, and then it lies in the row */
@@ -362,9 +390,9 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                 for (int k = 0; k < 3; k++) {
                     const int rr = clampi(rc + k - 1, 0, 7);
                     const uint8_t *q = rbase + (size_t)rr * L.stride;
-                    rw[k][0] = *(const uint32_t *)(q + a);
-                    rw[k][1] = *(const uint32_t *)(q + a + 4);
-                    rw[k][2] = *(const uint32_t *)(q + a2);
+                    rw[k][0] = load_global<uint32_t>(q + a);
+                    rw[k][1] = load_global<uint32_t>(q + a + 4);
+                    rw[k][2] = load_global<uint32_t>(q + a2);
                 }
                 /* the 8 reference pels of rows rc-1, rc, rc+1 */
                 int a8[3][8];
@@ -407,8 +435,8 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                 int res[8];
 #pragma unroll
                 for (int x = 0; x < 4; x++) {
-                    res[x] = (int)((cw.x >> (8 * x)) & 0xff) - pred[x];
-                    res[x + 4] = (int)((cw.y >> (8 * x)) & 0xff) - pred[x + 4];
+                    res[x] = (int)((cw0 >> (8 * x)) & 0xff) - pred[x];
+                    res[x + 4] = (int)((cw1 >> (8 * x)) & 0xff) - pred[x + 4];
                     predw[0] |= (uint32_t)pred[x] << (8 * x);
                     predw[1] |= (uint32_t)pred[x + 4] << (8 * x);
                 }
@@ -491,7 +519,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             if (L.act) {
                 uint8_t *dst = info[L.mi].rec[L.plane] + (size_t)L.py * L.stride + L.px;
                 if (!coded) {
-                    *(uint2 *)(dst + (size_t)rc * L.stride) = make_uint2(predw[0], predw[1]);
+                    store_global2(dst + (size_t)rc * L.stride, predw[0], predw[1]);
                 } else {
                     pred_lds[bi * 8 + rc] = make_uint2(predw[0], predw[1]);
                 }
@@ -565,7 +593,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     w[x >> 2] |= (uint32_t)val << (8 * (x & 3));
                 }
                 uint8_t *dst = info[C.mi].rec[C.plane] + (size_t)(C.py + rc) * C.stride + C.px;
-                *(uint2 *)dst = make_uint2(w[0], w[1]);
+                store_global2(dst, w[0], w[1]);
             }
             __syncthreads(); /* tf and slots are reused by the next pass */
         }

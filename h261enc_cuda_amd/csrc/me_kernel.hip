@@ -283,6 +283,14 @@ __device__ __forceinline__ void sad16(const uint32_t *win, const uint8_t *cur_mb
 
 typedef short short2v __attribute__((ext_vector_type(2)));
 
+/* a plane pointer from a stream descriptor is a generic pointer to the
+ * compiler; the load names the global address space, so it is not issued as
+ * a flat load (which would also count against the LDS wait counter) */
+__device__ __forceinline__ uint32_t load_global_u32(const uint8_t *p)
+{
+    return *(const __attribute__((address_space(1))) uint32_t *)(p);
+}
+
 __device__ __forceinline__ short2v as_s2(uint32_t v)
 {
     short2v r;
@@ -415,8 +423,8 @@ __device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc 
      * first so the load overlaps the staging */
     const int sub = lane >> 2, yq = lane & 3, sub_x = sub & 3, sub_y = sub >> 2;
     const int R = 4 * sub_y + yq; /* MB row */
-    const uint32_t crow = *(const uint32_t *)(cur_mb + R * W + 4 * sub_x);
-    const uint32_t cw = kVariant == kMeSurface ? *(const uint32_t *)(cur_mb + (lane >> 2) * W + 4 * (lane & 3)) : 0u;
+    const uint32_t crow = load_global_u32(cur_mb + R * W + 4 * sub_x);
+    const uint32_t cw = kVariant == kMeSurface ? load_global_u32(cur_mb + (lane >> 2) * W + 4 * (lane & 3)) : 0u;
 
     /* stage the window: all loads issued before any wait.  Lane (c4, rr) of
      * lanes 0-59 loads dword c4 of rows rr, rr + 5, ... (row 45 only for
