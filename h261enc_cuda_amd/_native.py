@@ -66,6 +66,7 @@ OPT_DEVICE_BITSTREAM = 6
 OPT_OVERLAP_WRITER = 7
 OPT_CHAINS = 8
 OPT_ME_MBS_PER_WAVE = 9
+OPT_ME_TILE = 10
 
 
 class StreamParams(ctypes.Structure):

@@ -1,6 +1,6 @@
 """Autotuner for the engine's launch shape.
 
-Three knobs change how the frame step's work is laid out on the device, not
+Four knobs change how the frame step's work is laid out on the device, not
 what it computes:
 
 * `mbs_per_wave` -- macroblocks one wave of the coder codes in sequence
@@ -9,6 +9,10 @@ what it computes:
 * `me_mbs_per_wave` -- consecutive macroblocks one motion-estimation wave
   searches in turn (1, 2, 4): fewer, longer waves, one cost atomic per
   stream segment of a run.
+* `me_tile` -- search-window tiling: macroblocks of a run that lie side by
+  side in one row share one staged reference tile of up to 1, 2 or 4
+  windows (at most the run).  A wider tile fetches each reference pel fewer
+  times but takes more LDS per wave.
 * `xcd_remap` -- bit 0 motion estimation, bit 1 coder, bit 2 bitstream
   writer: whether each XCD gets a contiguous run of macroblocks (neighbours
   share that XCD's L2) or workgroup i codes macroblock i.
@@ -16,9 +20,9 @@ what it computes:
 The tuner times candidates on a sample of the target workload -- the same
 mix of stream sizes, all streams active, inter frames -- with HIP events
 around every launch of the step, and keeps the fastest: first over
-`mbs_per_wave` at the default remap, then over `me_mbs_per_wave`, then over
-the remap at the best of both (coordinate descent; the knobs act on mostly
-different kernels).  Among remaps within `tie` (1%) of the fastest, the fastest that
+`mbs_per_wave` at the default remap, then over the pairs (`me_mbs_per_wave`,
+`me_tile`), then over the remap at the best of both (coordinate descent; the
+knobs act on mostly different kernels).  Among remaps within `tie` (1%) of the fastest, the fastest that
 remaps motion estimation is preferred: its time is the same within noise, and it
 reads about 7x fewer HBM bytes (neighbouring MBs' search windows meet in one
 XCD's L2), which leaves the memory system to concurrent work.  One engine
@@ -41,17 +45,19 @@ def _step_ms(t: dict) -> float:
 
 
 def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 2, 4),
-         xcd_remap: Iterable[int] = tuple(range(8)), me_mbs_per_wave: Iterable[int] = (1, 2, 4), steps: int = 4,
-         seed: int = 99, tie: float = 0.01) -> dict:
+         xcd_remap: Iterable[int] = tuple(range(8)), me_mbs_per_wave: Iterable[int] = (1, 2, 4),
+         me_tile: Iterable[int] = (1, 2, 4), steps: int = 4, seed: int = 99, tie: float = 0.01) -> dict:
     """sizes: (width, height) of every stream of the sample.  Returns
-    {"mbs_per_wave": best, "me_mbs_per_wave": best, "xcd_remap": best,
-    "step_ms": its time per step, "trials": [{"mbs_per_wave",
-    "me_mbs_per_wave", "xcd_remap", "step_ms", kernel times}]}."""
-    mpw_c, remap_c, me_c = list(mbs_per_wave), list(xcd_remap), list(me_mbs_per_wave)
-    key = (tuple(sizes), tuple(mpw_c), tuple(remap_c), tuple(me_c), steps)
+    {"mbs_per_wave": best, "me_mbs_per_wave": best, "me_tile": best,
+    "xcd_remap": best, "step_ms": its time per step, "trials":
+    [{"mbs_per_wave", "me_mbs_per_wave", "me_tile", "xcd_remap", "step_ms",
+    kernel times}]}.  A tile wider than the run is not tried."""
+    mpw_c, remap_c, me_c, tile_c = list(mbs_per_wave), list(xcd_remap), list(me_mbs_per_wave), list(me_tile)
+    me_shapes = [(m, t) for m in me_c for t in tile_c if t <= max(m, 1)]
+    key = (tuple(sizes), tuple(mpw_c), tuple(remap_c), tuple(me_shapes), steps)
     if key in _CACHE:
         return _CACHE[key]
-    frames = 2 + (steps + 1) * (len(mpw_c) + len(remap_c) + len(me_c))
+    frames = 2 + (steps + 1) * (len(mpw_c) + len(remap_c) + len(me_shapes))
     eng = Engine(max_streams=len(sizes))
     # kernels alone (the writer at the end of its step, not beside the next
     # step's motion estimation), so the event spans add up to the step
@@ -65,10 +71,11 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
         eng.step()
         eng.sync()
 
-        def trial(mpw: int, remap: int, me_mpw: int = 0) -> float:
+        def trial(mpw: int, remap: int, me_mpw: int = 0, tile: int = 0) -> float:
             eng.config(nat.OPT_MBS_PER_WAVE, mpw)
             eng.config(nat.OPT_XCD_REMAP, remap)
             eng.config(nat.OPT_ME_MBS_PER_WAVE, me_mpw)
+            eng.config(nat.OPT_ME_TILE, tile)
             eng.step()
             eng.sync()
             eng.profile(True)
@@ -78,7 +85,8 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
             eng.profile(False)
             ms = _step_ms(t)
             n = max(1, t["steps"])
-            trials.append({"mbs_per_wave": mpw, "me_mbs_per_wave": me_mpw, "xcd_remap": remap, "step_ms": round(ms, 4),
+            trials.append({"mbs_per_wave": mpw, "me_mbs_per_wave": me_mpw, "me_tile": tile, "xcd_remap": remap,
+                           "step_ms": round(ms, 4),
                            "me_ms": round(t["me_ms"] / n, 4), "mb_ms": round(t["mb_ms"] / n, 4),
                            "bits_ms": round(t["bits_ms"] / n, 4)})
             return ms
@@ -92,9 +100,9 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
         mpw_close = [m for m in mpw_c if mpw_t[m] <= mpw_fast * (1 + tie)]
         best_mpw = min((m for m in mpw_close if m), key=lambda m: mpw_t[m],
                        default=min(mpw_close, key=lambda m: mpw_t[m]))
-        me_t = {m: trial(best_mpw, default_remap, m) for m in me_c}
-        best_me = min(me_c, key=lambda m: me_t[m])
-        times = {r: trial(best_mpw, r, best_me) for r in remap_c}
+        me_t = {sh: trial(best_mpw, default_remap, *sh) for sh in me_shapes}
+        best_me, best_tile = min(me_shapes, key=lambda sh: me_t[sh])
+        times = {r: trial(best_mpw, r, best_me, best_tile) for r in remap_c}
         fastest = min(times.values())
         close = [r for r in remap_c if times[r] <= fastest * (1 + tie)]
         best_remap = min((r for r in close if r & 1), key=lambda r: times[r],
@@ -102,9 +110,9 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
     finally:
         eng.destroy()
     best = min((t for t in trials if t["mbs_per_wave"] == best_mpw and t["xcd_remap"] == best_remap
-                and t["me_mbs_per_wave"] == best_me), key=lambda t: t["step_ms"])
-    out = {"mbs_per_wave": best_mpw, "me_mbs_per_wave": best_me, "xcd_remap": best_remap, "step_ms": best["step_ms"],
-           "trials": trials}
+                and t["me_mbs_per_wave"] == best_me and t["me_tile"] == best_tile), key=lambda t: t["step_ms"])
+    out = {"mbs_per_wave": best_mpw, "me_mbs_per_wave": best_me, "me_tile": best_tile, "xcd_remap": best_remap,
+           "step_ms": best["step_ms"], "trials": trials}
     _CACHE[key] = out
     return out
 
@@ -114,3 +122,4 @@ def apply(eng: Engine, tuned: dict) -> None:
     eng.config(nat.OPT_MBS_PER_WAVE, tuned["mbs_per_wave"])
     eng.config(nat.OPT_XCD_REMAP, tuned["xcd_remap"])
     eng.config(nat.OPT_ME_MBS_PER_WAVE, tuned.get("me_mbs_per_wave", 0))
+    eng.config(nat.OPT_ME_TILE, tuned.get("me_tile", 0))

@@ -171,6 +171,7 @@ struct h261g_engine {
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_mbs_per_wave = 0; /* motion estimation: 0 chosen per launch, else 1, 2, 4 or 8 */
+    int me_tile = 0;         /* MBs per staged search-window tile: 0 chosen with the run, else 1, 2 or 4 */
     int me_variant = kMeTwoStage;
     /* 1: rate-control begin in the ME tail; 2: its own launch; 3: its own
      * launch when a chain fills the device (a short step is latency bound),
@@ -425,6 +426,7 @@ constexpr int kFillWaves = 256 * 4 * 5;
  * the MBs per wave of a launch that holds several times as many MBs */
 constexpr int kMeResidentWaves = 256 * 4 * 7;
 constexpr int kMeRunDefault = 2;
+constexpr int kMeTileDefault = 2;
 
 /* one frame step: three launches (default), or the six standalone launches */
 /* the bitstream writer over the whole MB space (MBs not placed by a frame
@@ -503,6 +505,7 @@ int enqueue_chain(h261g_engine *e, hipStream_t st, int lo, int hi, bool whole, b
     ml.fuse_tail = fused == 1;
     ml.xcd_remap = e->xcd_remap & 1;
     ml.mbs_per_wave = me_mbs_per_wave(e, hi - lo);
+    ml.tile = e->me_tile ? e->me_tile : min(ml.mbs_per_wave, kMeTileDefault);
     /* one chain: the previous step's bitstreams beside this step's motion
      * estimation (with nothing pending every MB is skipped) */
     const bool overlap =
@@ -674,6 +677,10 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
     case H261G_OPT_ME_MBS_PER_WAVE:
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return H261G_ERR_ARG;
         e->me_mbs_per_wave = value;
+        break;
+    case H261G_OPT_ME_TILE:
+        if (value != 0 && value != 1 && value != 2 && value != 4) return H261G_ERR_ARG;
+        e->me_tile = value;
         break;
     case H261G_OPT_DEVICE_BITSTREAM:
         if (value < 0 || value > 2) return H261G_ERR_ARG;

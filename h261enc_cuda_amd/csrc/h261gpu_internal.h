@@ -174,6 +174,8 @@ struct MeLaunch {
                                      0: costs only accumulate (rate-control begin is launched) */
     int32_t xcd_remap;            /* 1: contiguous macroblock runs per XCD (xcd_logical) */
     int32_t mbs_per_wave;         /* consecutive MBs one wave searches in turn (1, 2, 4 or 8; 0 = 1) */
+    int32_t tile;                 /* MBs of a run that share one staged window tile (1, 2 or 4; at most
+                                     mbs_per_wave; 0 = 1) */
 };
 enum : int32_t {
     kMeTwoStage = 0, /* the reference's candidates only, one v_sad_u8 lane per candidate */

@@ -1,9 +1,12 @@
 /*
  * me_kernel.hip -- motion estimation for gfx950.
  *
- * One wave64 per 16x16 macroblock:
+ * One wave64 per 16x16 macroblock, a run of consecutive MBs per wave:
  *   1. the reference search window (rows by-15..by+30, cols bx-16..bx+31)
- *      is staged in LDS with aligned dword loads, the current MB next to it;
+ *      is staged in LDS with aligned dword loads.  MBs of a run that lie
+ *      side by side in one MB row share one staged tile of kTile windows
+ *      (rows by-15..by+30, cols bx-16..bx+16 kTile+15): each reference pel
+ *      is fetched once per tile instead of up to three times;
  *   2. stage 1 evaluates the 11x11 coarse grid
  *      {-14,-11,-8,-5,-2,0,2,5,8,11,14}^2 (src/megpu_kernels.cuh:29-32),
  *      stage 2 the 7x7 refine around the winner (megpu:309-312, 338-339),
@@ -37,29 +40,38 @@ constexpr int kWinRows = 46;  /* 16 + 2*15 */
 /* word 3 of a gfx9 buffer resource: 32-bit data format, raw addressing */
 constexpr int kBufferDword3 = 0x00020000;
 constexpr int kWinDw = 12;    /* 48 bytes per row: frame cols bx-16 .. bx+31 */
-/* LDS dwords per window row: 15 for the two-stage search (with the lanes of
- * a coarse or refine pass spread over rows 2-3 apart, a 15-dword pitch puts
- * their reads in distinct banks more often than 12 does), 12 for the surface
- * variant, whose 16-byte row loads need aligned rows */
-template <int kVariant>
+/* dwords per tile row: kTile windows side by side, 16 pels apart */
+constexpr int tile_dw(int tile)
+{
+    return 4 * tile + 8;
+}
+/* LDS dwords per tile row.  Two-stage search: an odd pitch whose residue
+ * mod 32 lets the lane tables below read without bank conflicts (15 for one
+ * window; a tile needs at least tile_dw dwords: 17 for two, 29 for four).
+ * Surface variant: 12, its 16-byte row loads need aligned rows. */
+template <int kVariant, int kTile>
 constexpr int win_pitch()
 {
-    return kVariant == 1 ? kWinDw : 15;
+    return kVariant == 1 ? kWinDw : kTile == 1 ? 15 : kTile == 2 ? 17 : 29;
 }
-/* The two-stage window has two spare rows above and below and one spare
+/* The two-stage tile has two spare rows above and below and one spare
  * dword in front: a refine candidate outside the limits (up to 17 px from
  * the MB) then still reads LDS inside the array; its cost is replaced by
  * 65535. win = winmem + win_base. */
-template <int kVariant>
+template <int kVariant, int kTile>
 constexpr int win_base()
 {
-    return kVariant == 1 ? 0 : 2 * win_pitch<kVariant>() + 1;
+    return kVariant == 1 ? 0 : 2 * win_pitch<kVariant, kTile>() + 1;
 }
-template <int kVariant>
+template <int kVariant, int kTile>
 constexpr int win_alloc()
 {
-    return kVariant == 1 ? kWinRows * kWinDw : (win_base<kVariant>() + (kWinRows + 2) * win_pitch<kVariant>() + 3) & ~3;
+    return kVariant == 1 ? kWinRows * kWinDw
+                         : (win_base<kVariant, kTile>() + (kWinRows + 2) * win_pitch<kVariant, kTile>() + 3) & ~3;
 }
+/* a refine read of the tile's last MB reaches dword tile_dw (for candidates
+ * outside the limits, whose cost is replaced): still inside the row */
+static_assert(tile_dw(1) < 15 && tile_dw(2) < 17 && tile_dw(4) < 29, "tile rows fit their pitch");
 
 /* minimum over the wave: quads, half rows and rows by DPP, rows by readlane */
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
@@ -112,78 +124,91 @@ constexpr uint32_t morton_c(uint32_t x, uint32_t y)
     return k;
 }
 
-/* Per-lane candidate tables of the two-stage search (window pitch 15).
- * A candidate (vx, vy) starts at LDS byte 4 * 15 * (vy + 15) + vx + 16 of
- * the window: its dword is that >> 2, its byte shift that & 3.
+/* Per-lane candidate tables of the two-stage search, one set per tile pitch
+ * P (15, 17, 29).  A candidate (vx, vy) starts at LDS byte
+ * 4 P (vy + 15) + vx + 16 of its MB's window: its dword is that >> 2, its
+ * byte shift that & 3.
  *
  * Lane layout.  LDS serves a wave's dword reads a half-wave (32 lanes) at a
  * time over 32 banks; lanes reading different dwords of one bank wait for
  * each other.  All lanes of a read add the same row and dword to their
- * candidate's start, so a half-wave reads without conflict when its
- * candidates' distinct start dwords lie in distinct banks.
- *   coarse: kCoarseOrder[2 i + h][j] is the candidate (tx + 11 ty) of lane
- *           32 h + j in slot i, an assignment of the 121 candidates found by
- *           a search for that property (88 distinct start dwords in 4 x 32
- *           bank slots; checked below).  Spare lanes repeat the first
- *           candidate of their half-wave (the same dword, so a broadcast)
- *           and are masked off;
+ * candidate's start (and all lanes the same window offset within the tile),
+ * so a half-wave reads without conflict when its candidates' distinct start
+ * dwords lie in distinct banks.
+ *   coarse: kCoarseOrder[P][2 i + h][j] is the candidate (tx + 11 ty) of
+ *           lane 32 h + j in slot i, an assignment of the 121 candidates
+ *           found by a search for that property (88 distinct start dwords
+ *           in 4 x 32 bank slots; checked below).  Spare lanes repeat the
+ *           first candidate of their half-wave (the same dword, so a
+ *           broadcast) and are masked off;
  *   refine: the 7 x 7 rows ty go to lanes 7 j + tx of half-wave 0 in the
- *           order 0, 1, 4, 5 and of half-wave 1 in the order 2, 3, 6: rows
- *           15 dwords apart then never meet in a bank, whatever the winner's
- *           byte alignment.  Spare lanes repeat the first lane of their
- *           half-wave and are masked off.
- * Entries: byte address (coarse) or 60 ty + tx added to the winner's byte
+ *           order 0, 1, 4, 5 and of half-wave 1 in the order 2, 3, 6: for
+ *           each pitch, rows P dwords apart then never meet in a bank,
+ *           whatever the winner's byte alignment.  Spare lanes repeat the
+ *           first lane of their half-wave and are masked off.
+ * Entries: byte address (coarse) or 4 P ty + tx added to the winner's byte
  * address (refine) | Z-order key << 16. */
-constexpr int kTwoStagePitch = 15;
-constexpr int8_t kCoarseOrder[4][32] = {
-    {0, 5, 6, 7, 13, 14, 16, 17, 24, 25, 26, 27, 28, 29, 32, 49, 50, 52, 53, 57, 58, 73, 74, 75, 76, 88, 89, 104, 105, -1, -1, -1},
-    {8, 9, 10, 11, 34, 35, 36, 43, 54, 55, 56, 60, 61, 62, 66, 67, 71, 72, 78, 85, 86, 100, 101, 102, 103, 107, 108, 110, 118, 119, 120, -1},
-    {4, 12, 18, 19, 20, 21, 23, 30, 31, 37, 48, 59, 63, 64, 65, 77, 82, 83, 84, 87, 90, 91, 92, 96, 97, 112, 113, 114, 117, -1, -1, -1},
-    {1, 2, 3, 15, 22, 33, 38, 39, 40, 41, 42, 44, 45, 46, 47, 51, 68, 69, 70, 79, 80, 81, 93, 94, 95, 98, 99, 106, 109, 111, 115, 116},
+constexpr int kTwoStagePitches[3] = {15, 17, 29};
+constexpr int pitch_set(int pitch)
+{
+    return pitch == 15 ? 0 : pitch == 17 ? 1 : 2;
+}
+constexpr int8_t kCoarseOrder[3][4][32] = {
+    {/* pitch 15 */
+     {0, 5, 6, 7, 13, 14, 16, 17, 24, 25, 26, 27, 28, 29, 32, 49, 50, 52, 53, 57, 58, 73, 74, 75, 76, 88, 89, 104, 105, -1, -1, -1},
+     {8, 9, 10, 11, 34, 35, 36, 43, 54, 55, 56, 60, 61, 62, 66, 67, 71, 72, 78, 85, 86, 100, 101, 102, 103, 107, 108, 110, 118, 119, 120, -1},
+     {4, 12, 18, 19, 20, 21, 23, 30, 31, 37, 48, 59, 63, 64, 65, 77, 82, 83, 84, 87, 90, 91, 92, 96, 97, 112, 113, 114, 117, -1, -1, -1},
+     {1, 2, 3, 15, 22, 33, 38, 39, 40, 41, 42, 44, 45, 46, 47, 51, 68, 69, 70, 79, 80, 81, 93, 94, 95, 98, 99, 106, 109, 111, 115, 116}},
+    {/* pitch 17 */
+     {4, 7, 10, 12, 16, 17, 24, 25, 26, 27, 28, 29, 41, 42, 56, 60, 61, 65, 66, 70, 74, 75, 92, 95, 106, 111, 112, 113, 115, 116, -1, -1},
+     {1, 5, 6, 13, 14, 15, 30, 31, 34, 37, 38, 39, 45, 46, 47, 48, 51, 62, 76, 77, 79, 80, 81, 82, 83, 90, 91, 99, 100, 109, 114, -1},
+     {0, 2, 3, 11, 19, 20, 22, 23, 32, 40, 44, 49, 50, 52, 53, 59, 63, 64, 84, 85, 86, 87, 88, 89, 93, 94, 96, 97, 101, 102, -1, -1},
+     {8, 9, 18, 21, 33, 35, 36, 43, 54, 55, 57, 58, 67, 68, 69, 71, 72, 73, 78, 98, 103, 104, 105, 107, 108, 110, 117, 118, 119, 120, -1, -1}},
+    {/* pitch 29 */
+     {0, 1, 4, 8, 9, 16, 17, 19, 20, 30, 31, 32, 37, 43, 52, 53, 56, 57, 58, 59, 67, 85, 86, 88, 89, 98, 109, 112, 113, 120, -1, -1},
+     {2, 3, 10, 11, 18, 22, 23, 24, 25, 27, 28, 38, 39, 41, 42, 45, 46, 47, 48, 51, 54, 60, 61, 70, 74, 75, 78, 81, 104, 105, -1, -1},
+     {5, 6, 7, 21, 34, 44, 62, 68, 69, 71, 72, 73, 77, 82, 83, 87, 90, 91, 93, 94, 100, 101, 102, 103, 106, 110, 114, 115, 116, 117, -1, -1},
+     {12, 13, 14, 15, 26, 29, 33, 35, 36, 40, 49, 50, 55, 63, 64, 65, 66, 76, 79, 80, 84, 92, 95, 96, 97, 99, 107, 108, 111, 118, 119, -1}},
 };
 constexpr int kRefineRowBase[7] = {0, 7, 32, 39, 14, 21, 46}; /* first lane of refine row ty */
 
-constexpr int coarse_start(int ci)
+constexpr int coarse_start(int pitch, int ci)
 {
-    return 4 * kTwoStagePitch * (coarse_vec(ci / 11) + 15) + coarse_vec(ci % 11) + 16;
+    return 4 * pitch * (coarse_vec(ci / 11) + 15) + coarse_vec(ci % 11) + 16;
 }
-constexpr int coarse_at(int i, int l) /* candidate of lane l in slot i, spare lanes resolved */
+constexpr int coarse_at(int ps, int i, int l) /* candidate of lane l in slot i, spare lanes resolved */
 {
-    const int c = kCoarseOrder[2 * i + l / 32][l % 32];
-    return c >= 0 ? c : kCoarseOrder[2 * i + l / 32][0];
+    const int c = kCoarseOrder[ps][2 * i + l / 32][l % 32];
+    return c >= 0 ? c : kCoarseOrder[ps][2 * i + l / 32][0];
 }
 struct CoarseLanes {
     uint32_t e[2][64];
-    uint64_t row_below[2][12]; /* lanes of slot i whose candidate has ty < t */
-    uint64_t col_below[2][12]; /* ... tx < t */
 };
-constexpr CoarseLanes make_coarse_lanes()
+constexpr CoarseLanes make_coarse_lanes(int ps)
 {
     CoarseLanes t{};
     for (int i = 0; i < 2; i++)
         for (int l = 0; l < 64; l++) {
-            const int ci = coarse_at(i, l), tx = ci % 11, ty = ci / 11;
-            t.e[i][l] = (uint32_t)coarse_start(ci) | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
-            if (kCoarseOrder[2 * i + l / 32][l % 32] < 0) continue;
-            for (int b = ty + 1; b < 12; b++) t.row_below[i][b] |= 1ull << l;
-            for (int b = tx + 1; b < 12; b++) t.col_below[i][b] |= 1ull << l;
+            const int ci = coarse_at(ps, i, l), tx = ci % 11, ty = ci / 11;
+            t.e[i][l] = (uint32_t)coarse_start(kTwoStagePitches[ps], ci) | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
         }
     return t;
 }
 /* every candidate in exactly one lane; distinct start dwords of a half-wave
  * in distinct banks */
-constexpr bool coarse_order_ok()
+constexpr bool coarse_order_ok(int ps)
 {
+    const int pitch = kTwoStagePitches[ps];
     int seen[121] = {};
     for (int k = 0; k < 4; k++)
         for (int j = 0; j < 32; j++) {
-            const int c = kCoarseOrder[k][j];
+            const int c = kCoarseOrder[ps][k][j];
             if (c < -1 || c > 120) return false;
             if (c >= 0) seen[c]++;
             for (int j2 = 0; j2 < j; j2++) {
-                const int c2 = kCoarseOrder[k][j2];
+                const int c2 = kCoarseOrder[ps][k][j2];
                 if (c < 0 || c2 < 0) continue;
-                const int a = coarse_start(c) >> 2, a2 = coarse_start(c2) >> 2;
+                const int a = coarse_start(pitch, c) >> 2, a2 = coarse_start(pitch, c2) >> 2;
                 if (a != a2 && a % 32 == a2 % 32) return false;
             }
         }
@@ -191,12 +216,12 @@ constexpr bool coarse_order_ok()
         if (seen[c] != 1) return false;
     return true;
 }
-static_assert(coarse_order_ok(), "coarse lane layout");
+static_assert(coarse_order_ok(0) && coarse_order_ok(1) && coarse_order_ok(2), "coarse lane layouts");
 
 struct RefineLanes {
     uint32_t e[64];
 };
-constexpr RefineLanes make_refine_lanes()
+constexpr RefineLanes make_refine_lanes(int ps)
 {
     RefineLanes t{};
     for (int l = 0; l < 64; l++) {
@@ -206,13 +231,13 @@ constexpr RefineLanes make_refine_lanes()
                 tx = l - kRefineRowBase[r];
                 ty = r;
             }
-        t.e[l] = (uint32_t)(4 * kTwoStagePitch * ty + tx) | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
+        t.e[l] = (uint32_t)(4 * kTwoStagePitches[ps] * ty + tx) | (morton_c((uint32_t)tx, (uint32_t)ty) << 16);
     }
     return t;
 }
-/* rows of a half-wave: start dwords 15 ty + (a + tx) / 4, a = 0..3, in
+/* rows of a half-wave: start dwords P ty + (a + tx) / 4, a = 0..3, in
  * distinct banks for distinct dwords */
-constexpr bool refine_rows_ok()
+constexpr bool refine_rows_ok(int pitch)
 {
     for (int a = 0; a < 4; a++)
         for (int r = 0; r < 7; r++)
@@ -220,15 +245,49 @@ constexpr bool refine_rows_ok()
                 for (int x = 0; x < 7; x++)
                     for (int x2 = 0; x2 < 7; x2++) {
                         if ((kRefineRowBase[r] < 32) != (kRefineRowBase[r2] < 32)) continue;
-                        const int d = kTwoStagePitch * r + (a + x) / 4, d2 = kTwoStagePitch * r2 + (a + x2) / 4;
+                        const int d = pitch * r + (a + x) / 4, d2 = pitch * r2 + (a + x2) / 4;
                         if (d != d2 && d % 32 == d2 % 32) return false;
                     }
     return kRefineRowBase[3] + 7 <= 64;
 }
-static_assert(refine_rows_ok(), "refine lane layout");
+static_assert(refine_rows_ok(15) && refine_rows_ok(17) && refine_rows_ok(29), "refine lane layout");
 
-__constant__ CoarseLanes c_coarse = make_coarse_lanes();
-__constant__ RefineLanes c_refine = make_refine_lanes();
+/* Lane masks of the in-limit coarse candidates by frame edge.  An MB's
+ * limits are +-15 clipped at the frame border (gpu_device.cu:226-265) and MB
+ * positions are multiples of 16, so each limit has one of two values: -15
+ * or 0 on the left and top, 15 or 0 on the right and bottom.  Entry
+ * (left | right << 1 | top << 2 | bottom << 3) holds the masks of the two
+ * coarse slots; spare lanes are never valid. */
+struct CoarseEdgeMasks {
+    uint64_t m[16][2];
+};
+constexpr CoarseEdgeMasks make_coarse_edge_masks(int ps)
+{
+    CoarseEdgeMasks t{};
+    for (int e = 0; e < 16; e++) {
+        const int lx0 = (e & 1) ? 0 : -15, lx1 = (e & 2) ? 0 : 15, ly0 = (e & 4) ? 0 : -15, ly1 = (e & 8) ? 0 : 15;
+        for (int i = 0; i < 2; i++)
+            for (int l = 0; l < 64; l++) {
+                const int c = kCoarseOrder[ps][2 * i + l / 32][l % 32];
+                if (c < 0) continue;
+                const int vx = coarse_vec(c % 11), vy = coarse_vec(c / 11);
+                if (vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1) t.m[e][i] |= 1ull << l;
+            }
+    }
+    return t;
+}
+
+struct TwoStageTables {
+    CoarseLanes coarse;
+    RefineLanes refine;
+    CoarseEdgeMasks edge;
+};
+constexpr TwoStageTables make_two_stage_tables(int ps)
+{
+    return TwoStageTables{make_coarse_lanes(ps), make_refine_lanes(ps), make_coarse_edge_masks(ps)};
+}
+__constant__ TwoStageTables c_two_stage[3] = {make_two_stage_tables(0), make_two_stage_tables(1),
+                                              make_two_stage_tables(2)};
 
 __device__ __forceinline__ uint64_t refine_mask(int sx, int sy, int lx0, int lx1, int ly0, int ly1)
 {
@@ -371,47 +430,114 @@ __device__ __forceinline__ void sad_surface(const uint32_t *win, const uint32_t 
     out[1] = make_uint4((uint32_t)acc2, (uint32_t)(acc2 >> 32), (uint32_t)acc3, (uint32_t)(acc3 >> 32));
 }
 
-/* Lane masks of the in-limit coarse candidates by frame edge.  An MB's
- * limits are +-15 clipped at the frame border (gpu_device.cu:226-265) and MB
- * positions are multiples of 16, so each limit has one of two values: -15
- * or 0 on the left and top, 15 or 0 on the right and bottom.  Entry
- * (left | right << 1 | top << 2 | bottom << 3) holds the masks of the two
- * coarse slots; spare lanes are never valid. */
-struct CoarseEdgeMasks {
-    uint64_t m[16][2];
+
+/* The staged tile of a wave: global MBs g .. end-1, side by side in one MB
+ * row of one stream (end = 0: nothing staged yet) */
+struct Tile {
+    int g, end;
 };
-constexpr CoarseEdgeMasks make_coarse_edge_masks()
+
+/* Stages the tile whose first MB is at (bx, by): rows by-15..by+30, dwords
+ * c = 0..D-1 of frame columns bx-16+4c (D = 4 kTile + 8; 12 for the surface
+ * variant).  All loads are issued before any wait: lane (c, rr) of the
+ * first RPI x D lanes loads dword c of rows rr, rr + RPI, ... (RPI = 64 / D
+ * rows per pass).  Tile dwords outside the frame come from a clamped
+ * in-frame address (no in-limit candidate or filter tap reads them).  When
+ * all 46 rows lie inside the frame, each staged row has a scalar base and
+ * the lane a 32-bit offset. */
+template <int kVariant, int W, int kTile, int kFirst, int kCount>
+__device__ __forceinline__ void stage_passes(const __amdgpu_buffer_rsrc_t rsrc, bool inside, int by, int fx, int c,
+                                             int rr, uint32_t *win)
 {
-    CoarseEdgeMasks t{};
-    for (int e = 0; e < 16; e++) {
-        const int lx0 = (e & 1) ? 0 : -15, lx1 = (e & 2) ? 0 : 15, ly0 = (e & 4) ? 0 : -15, ly1 = (e & 8) ? 0 : 15;
-        for (int i = 0; i < 2; i++)
-            for (int l = 0; l < 64; l++) {
-                const int c = kCoarseOrder[2 * i + l / 32][l % 32];
-                if (c < 0) continue;
-                const int vx = coarse_vec(c % 11), vy = coarse_vec(c / 11);
-                if (vx >= lx0 && vx <= lx1 && vy >= ly0 && vy <= ly1) t.m[e][i] |= 1ull << l;
+    constexpr int H = W == 352 ? 288 : 144;
+    constexpr int D = kVariant == kMeSurface ? kWinDw : tile_dw(kTile);
+    constexpr int kPitch = win_pitch<kVariant, kTile>();
+    constexpr int RPI = 64 / D;
+    constexpr int kFull = kWinRows / RPI; /* passes with every row inside the tile */
+    uint32_t staged[kCount];
+    if (inside) {
+        const uint32_t lo = (uint32_t)(rr * W + fx);
+#pragma unroll
+        for (int i = 0; i < kCount; i++) {
+            const int t = kFirst + i;
+            if (t < kFull) {
+                staged[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, lo, (by - 15 + RPI * t) * W, 0);
+            } else {
+                const int fy = by - 15 + min(RPI * t + rr, kWinRows - 1);
+                staged[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)(fy * W + fx), 0, 0);
             }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kCount; i++) {
+            const int t = kFirst + i;
+            const int fy = min(max(by - 15 + min(RPI * t + rr, kWinRows - 1), 0), H - 1);
+            staged[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)(fy * W + fx), 0, 0);
+        }
     }
-    return t;
+    if (threadIdx.x < RPI * D) {
+#pragma unroll
+        for (int i = 0; i < kCount; i++) {
+            const int t = kFirst + i;
+            if (t < kFull || RPI * t + rr < kWinRows) win[(RPI * t + rr) * kPitch + c] = staged[i];
+        }
+    }
 }
-__constant__ CoarseEdgeMasks c_coarse_edge = make_coarse_edge_masks();
+
+/* Stages the tile whose first MB is at (bx, by): rows by-15..by+30, dwords
+ * c = 0..D-1 of frame columns bx-16+4c (D = 4 kTile + 8; 12 for the surface
+ * variant).  Lane (c, rr) of the first RPI x D lanes loads dword c of rows
+ * rr, rr + RPI, ... (RPI = 64 / D rows per pass); the loads of up to 12
+ * passes are issued before any wait (a four-MB tile takes 23 passes, in
+ * batches of 8, so that the staged values do not spill).  Tile dwords
+ * outside the frame come from a clamped in-frame address (no in-limit
+ * candidate or filter tap reads them).  When all 46 rows lie inside the
+ * frame, each staged row has a scalar base and the lane a 32-bit offset. */
+template <int kVariant, int W, int kTile>
+__device__ __forceinline__ void stage_tile(const uint8_t *ref, int bx, int by, uint32_t *win)
+{
+    constexpr int H = W == 352 ? 288 : 144;
+    constexpr int D = kVariant == kMeSurface ? kWinDw : tile_dw(kTile);
+    constexpr int RPI = 64 / D;                        /* 5, 4 or 2 */
+    constexpr int kIters = (kWinRows + RPI - 1) / RPI; /* 10, 12 or 23 */
+    constexpr int kBatch = kIters <= 12 ? kIters : 8;
+    const int lane = threadIdx.x;
+    const int c = lane % D, rr = min(lane / D, RPI - 1);
+    const int fx = min(max(bx - 16 + 4 * c, 0), W - 4);
+    /* buffer loads over the luma plane: a 32-bit lane offset, and in the
+     * interior case a scalar row offset (addresses stay in the plane, so its
+     * range check never applies).  The plane's address is wave-uniform: said
+     * so explicitly, else the resource is built per lane and each load loops
+     * over lanes */
+    const uint64_t ref_u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uintptr_t)ref >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uintptr_t)ref);
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)ref_u, 0, W * H, kBufferDword3);
+    const bool inside = by >= 16 && by + 32 <= H;
+    stage_passes<kVariant, W, kTile, 0, kBatch>(rsrc, inside, by, fx, c, rr, win);
+    if constexpr (kIters > kBatch)
+        stage_passes<kVariant, W, kTile, kBatch, (kIters - kBatch < kBatch ? kIters - kBatch : kBatch)>(rsrc, inside, by, fx, c, rr, win);
+    if constexpr (kIters > 2 * kBatch)
+        stage_passes<kVariant, W, kTile, 2 * kBatch, kIters - 2 * kBatch>(rsrc, inside, by, fx, c, rr, win);
+    static_assert(kIters <= 3 * kBatch, "three batches at most");
+}
 
 /* One macroblock of a W-pel-wide frame (352 CIF, 176 QCIF: every size and
  * divisor below is a compile-time constant, so MB positions, plane offsets
- * and the current MB's scalar row loads need no run-time arithmetic). */
-template <int kVariant, int W>
-__device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc &d, uint32_t *win, uint32_t *curw,
-                                     uint16_t *surf)
+ * and the current MB's scalar row loads need no run-time arithmetic).  The
+ * MB's window is the tile's columns from 16 (mx - tile.mx) pels on; a new
+ * tile is staged when the MB lies outside the current one. */
+template <int kVariant, int W, int kTile>
+__device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc &d, uint32_t *tile_win,
+                                     uint32_t *curw, uint16_t *surf, Tile &tile)
 {
     constexpr int H = W == 352 ? 288 : 144, MBW = W / 16, MBH = H / 16;
-    constexpr int kPitch = win_pitch<kVariant>();
+    constexpr int kPitch = win_pitch<kVariant, kTile>();
     const int lane = threadIdx.x;
     const int m = g - d.mb_base;
     const int mx = m % MBW, my = m / MBW;
     const int bx = 16 * mx, by = 16 * my;
     const uint8_t *cur = d.cur[0];
-    const uint8_t *ref = d.ref[0];
     const uint8_t *cur_mb = cur + by * W + bx;
     const int edges = (mx == 0 ? 1 : 0) | (mx == MBW - 1 ? 2 : 0) | (my == 0 ? 4 : 0) | (my == MBH - 1 ? 8 : 0);
 
@@ -426,45 +552,14 @@ __device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc 
     const uint32_t crow = load_global_u32(cur_mb + R * W + 4 * sub_x);
     const uint32_t cw = kVariant == kMeSurface ? load_global_u32(cur_mb + (lane >> 2) * W + 4 * (lane & 3)) : 0u;
 
-    /* stage the window: all loads issued before any wait.  Lane (c4, rr) of
-     * lanes 0-59 loads dword c4 of rows rr, rr + 5, ... (row 45 only for
-     * rr = 0); window dwords outside the frame come from a clamped in-frame
-     * address (no in-limit candidate or filter tap reads them).  When all 46
-     * rows lie inside the frame, each staged row has a scalar base and the
-     * lane a 32-bit offset. */
-    constexpr int kStageIters = (kWinRows + 4) / 5; /* 10 */
-    const int c4 = lane % kWinDw, rr = min(lane / kWinDw, 4);
-    const int fx = min(max(bx - 16 + 4 * c4, 0), W - 4);
-    uint32_t staged[kStageIters];
-    /* buffer loads over the luma plane: a 32-bit lane offset, and in the
-     * interior case a scalar row offset (addresses stay in the plane, so its
-     * range check never applies).  The plane's address is wave-uniform: said
-     * so explicitly, else the resource is built per lane and each load loops
-     * over lanes */
-    const uint64_t ref_u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uintptr_t)ref >> 32)) << 32) |
-                           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uintptr_t)ref);
-    const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)ref_u, 0, W * H, kBufferDword3);
-    if (!(edges & 12)) {
-        const uint32_t lo = (uint32_t)(rr * W + fx);
-#pragma unroll
-        for (int t = 0; t < kStageIters - 1; t++)
-            staged[t] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, lo, (by - 15 + 5 * t) * W, 0);
-        staged[kStageIters - 1] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)fx, (by + 30) * W, 0);
-    } else {
-#pragma unroll
-        for (int t = 0; t < kStageIters; t++) {
-            const int fy = min(max(by - 15 + (t < kStageIters - 1 ? rr : 0) + 5 * t, 0), H - 1);
-            staged[t] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)(fy * W + fx), 0, 0);
-        }
+    if (g >= tile.end) {
+        if (tile.end) __syncthreads(); /* the previous MBs' reads of the tile are done */
+        stage_tile<kVariant, W, kTile>(d.ref[0], bx, by, tile_win);
+        tile = Tile{g, g + min(kTile, MBW - mx)};
+        if (kVariant == kMeSurface) curw[lane] = cw;
+        __syncthreads();
     }
-    if (lane < 5 * kWinDw) {
-#pragma unroll
-        for (int t = 0; t < kStageIters - 1; t++) win[(rr + 5 * t) * kPitch + c4] = staged[t];
-        if (rr == 0) win[(kWinRows - 1) * kPitch + c4] = staged[kStageIters - 1];
-    }
-    if (kVariant == kMeSurface) curw[lane] = cw;
-    __syncthreads();
+    uint32_t *const win = tile_win + 4 * (g - tile.g);
 
     const bool search = !(p.skip_search_on_intra && d.frame_type == H261_FRAME_TYPE_INTRA);
     int mvx = 0, mvy = 0;
@@ -491,12 +586,12 @@ __device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc 
             /* every coarse candidate lies inside the window, so each is
              * evaluated where it is; the mask bit of an out-of-limit one (or
              * of a spare lane) replaces its cost by 65535 */
-            static_assert(kVariant == kMeSurface || kPitch == kTwoStagePitch, "lane tables assume the two-stage pitch");
+            const TwoStageTables &tab = c_two_stage[pitch_set(kPitch)];
             int off[2], sh[2];
             uint32_t e[2];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                e[i] = c_coarse.e[i][lane];
+                e[i] = tab.coarse.e[i][lane];
                 off[i] = (int)(e[i] & 0xffffu) >> 2;
                 sh[i] = (int)(e[i] & 3u);
             }
@@ -504,7 +599,7 @@ __device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc 
             sad16<2, kPitch>(win, cur_mb, W, off, sh, acc);
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                const uint32_t cost = __builtin_amdgcn_inverse_ballot_w64(c_coarse_edge.m[edges][i]) ? acc[i] : 65535u;
+                const uint32_t cost = __builtin_amdgcn_inverse_ballot_w64(tab.edge.m[edges][i]) ? acc[i] : 65535u;
                 const uint32_t key = (cost << 8) | (e[i] >> 16);
                 best = key < best ? key : best;
             }
@@ -526,7 +621,7 @@ __device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc 
              * read the window's spare rows and are masked off, as are the
              * spare lanes */
             const uint64_t m2 = refine_mask(sx, sy, lx0, lx1, ly0, ly1);
-            const uint32_t e2 = c_refine.e[lane];
+            const uint32_t e2 = c_two_stage[pitch_set(kPitch)].refine.e[lane];
             const int a = 4 * kPitch * (sy + 12) + (sx + 13) + (int)(e2 & 0xffffu);
             int off[1] = {a >> 2}, sh[1] = {a & 3};
             uint32_t acc[1];
@@ -639,15 +734,17 @@ __device__ __forceinline__ void add_costs(const MeLaunch &p, int s, const Stream
 
 /* One wave searches kMbs consecutive MBs of the global MB space in turn (the
  * launch's MB range in runs of kMbs): the wave's start-up and the tail of
- * each MB's stores and atomics overlap the next MB's work, and a run's MBs
- * of one stream add their costs with one atomic. */
-template <int kVariant, int kMbs>
+ * each MB's stores and atomics overlap the next MB's work, a run's MBs of
+ * one stream add their costs with one atomic, and up to kTile MBs of a run
+ * that lie side by side in one MB row search one staged tile. */
+template <int kVariant, int kMbs, int kTile>
 __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
 {
-    __shared__ __align__(16) uint32_t winmem[win_alloc<kVariant>()];
+    static_assert(kTile == 1 || (kVariant == kMeTwoStage && kTile <= kMbs), "tiles: two-stage search, within a run");
+    __shared__ __align__(16) uint32_t winmem[win_alloc<kVariant, kTile>()];
     __shared__ __align__(16) uint32_t curw[kVariant == kMeSurface ? 64 : 4];
     __shared__ __align__(16) uint16_t surf[kVariant == kMeSurface ? 31 * 32 : 8];
-    uint32_t *const win = winmem + win_base<kVariant>();
+    uint32_t *const win = winmem + win_base<kVariant, kTile>();
 
     const int runs = (p.n_mb - p.mb_first + kMbs - 1) / kMbs;
     const int r = launch_index(blockIdx.x, runs, p.xcd_remap);
@@ -656,6 +753,7 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
     int seg = -1, count = 0;
     uint32_t cost = 0;
     const StreamDesc *seg_d = nullptr;
+    Tile tile{0, 0};
 #pragma unroll 1
     for (int i = 0; i < kMbs; i++) {
         const int g = g0 + i;
@@ -669,9 +767,8 @@ __global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
             seg_d = s >= 0 ? &p.streams[s] : nullptr;
         }
         if (s < 0 || !(seg_d->flags & kStreamActive)) continue;
-        if (count) __syncthreads(); /* the previous MB's window reads are done */
-        cost += (uint32_t)(seg_d->width == 352 ? me_mb<kVariant, 352>(p, g, *seg_d, win, curw, surf)
-                                               : me_mb<kVariant, 176>(p, g, *seg_d, win, curw, surf));
+        cost += (uint32_t)(seg_d->width == 352 ? me_mb<kVariant, 352, kTile>(p, g, *seg_d, win, curw, surf, tile)
+                                               : me_mb<kVariant, 176, kTile>(p, g, *seg_d, win, curw, surf, tile));
         count++;
     }
     if (seg >= 0) add_costs(p, seg, *seg_d, count, cost);
@@ -689,13 +786,20 @@ int launch_me(const MeLaunch &p, void *stream, void *ev_start, void *ev_stop)
         return 0;
     }
     /* the surface variant searches one MB per wave (its LDS surface and
-     * registers leave no room for a run) */
+     * registers leave no room for a run); a tile is at most a run wide */
     const int mbs = p.variant == kMeSurface ? 1
                     : p.mbs_per_wave == 8 ? 8 : p.mbs_per_wave == 4 ? 4 : p.mbs_per_wave == 2 ? 2 : 1;
+    const int tile = min(p.tile == 4 ? 4 : p.tile == 2 ? 2 : 1, mbs);
     const dim3 grid(launch_grid((p.n_mb - p.mb_first + mbs - 1) / mbs, p.xcd_remap));
-    auto k = p.variant == kMeSurface ? me_kernel<kMeSurface, 1>
-             : mbs == 8 ? me_kernel<kMeTwoStage, 8> : mbs == 4 ? me_kernel<kMeTwoStage, 4>
-             : mbs == 2 ? me_kernel<kMeTwoStage, 2> : me_kernel<kMeTwoStage, 1>;
+    /* [log2 run][log2 tile] */
+    static void (*const two_stage[4][3])(MeLaunch) = {
+        {me_kernel<kMeTwoStage, 1, 1>, nullptr, nullptr},
+        {me_kernel<kMeTwoStage, 2, 1>, me_kernel<kMeTwoStage, 2, 2>, nullptr},
+        {me_kernel<kMeTwoStage, 4, 1>, me_kernel<kMeTwoStage, 4, 2>, me_kernel<kMeTwoStage, 4, 4>},
+        {me_kernel<kMeTwoStage, 8, 1>, me_kernel<kMeTwoStage, 8, 2>, me_kernel<kMeTwoStage, 8, 4>},
+    };
+    auto k = p.variant == kMeSurface ? me_kernel<kMeSurface, 1, 1>
+                                     : two_stage[mbs == 8 ? 3 : mbs == 4 ? 2 : mbs == 2 ? 1 : 0][tile == 4 ? 2 : tile - 1];
     if (ev_start)
         hipExtLaunchKernelGGL(k, grid, dim3(64), 0, (hipStream_t)stream, (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0,
                               p);

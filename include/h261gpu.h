@@ -157,6 +157,11 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
 #define H261G_OPT_ME_MBS_PER_WAVE 9 /* consecutive macroblocks one motion-estimation wave searches in
                                        turn: 1, 2, 4 or 8; 0 (default): chosen per launch from its
                                        size (1 while a launch is short) */
+#define H261G_OPT_ME_TILE 10 /* search-window tiling: consecutive macroblocks of a motion-estimation
+                                wave's run that lie side by side in one macroblock row share one
+                                staged reference tile of up to this many search windows: 1
+                                (every macroblock stages its own window), 2 or 4 (at most the
+                                run); 0 (default): chosen with the run */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */

@@ -338,16 +338,19 @@ def test_engine_me_runs_agree(oracle, fused):
     """Motion estimation with 1, 2, 4 or 8 consecutive MBs per wave (runs
     that cross stream boundaries, one cost atomic per stream segment, and in
     the fused form the wave completing a stream's count starting its
-    picture) codes the same bytes as the oracle."""
-    from h261enc_cuda_amd._native import OPT_FUSED_TAILS, OPT_ME_MBS_PER_WAVE
+    picture) and search-window tiles of 1, 2 or 4 MBs (tiles cut at row and
+    stream ends, at the right frame edge, at the top and bottom rows) codes
+    the same bytes as the oracle."""
+    from h261enc_cuda_amd._native import OPT_FUSED_TAILS, OPT_ME_MBS_PER_WAVE, OPT_ME_TILE
     specs = [(QCIF, 311), (CIF, 312), (QCIF, 313), (CIF, 314), (QCIF, 315)]
     n = 4
     results = []
-    for runs in (1, 2, 4, 8):
-        announce(f"motion estimation {runs} MBs per wave, fused={fused}")
+    for runs, tile in ((1, 1), (2, 1), (2, 2), (4, 1), (4, 2), (4, 4), (8, 2), (8, 4)):
+        announce(f"motion estimation {runs} MBs per wave, tile {tile}, fused={fused}")
         eng = Engine(max_streams=len(specs))
         eng.config(OPT_FUSED_TAILS, fused)
         eng.config(OPT_ME_MBS_PER_WAVE, runs)
+        eng.config(OPT_ME_TILE, tile)
         ids = []
         for size, seed in specs:
             sid = eng.open(*size, bitrate=96000, vbv_size=96000)

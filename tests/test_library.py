@@ -37,6 +37,15 @@ def test_legacy_prototypes_are_the_reference_sixteen():
         "h261_gpu_device_decode_macroblocks"])
 
 
+def test_option_numbers_match_the_header():
+    """Every H261G_OPT_* of include/h261gpu.h has its OPT_* twin in the
+    Python binding with the same number, and no other."""
+    text = open(os.path.join(ROOT, "include", "h261gpu.h")).read()
+    header = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define H261G_OPT_([A-Z0-9_]+)\s+(\d+)", text)}
+    py = {k[4:]: v for k, v in vars(_native).items() if k.startswith("OPT_") and isinstance(v, int)}
+    assert header and header == py
+
+
 def test_library_is_built_for_gfx950():
     """The shared object carries gfx950 device code objects."""
     blob = open(_native.LIB_PATH, "rb").read()
