@@ -164,9 +164,12 @@ struct h261g_engine {
     int overlap_writer = 2; /* 0 off, 1 on, 2 when a launch fills the device */
     bool writer_pending = false;
     /* two chains: the second half of the MB space is stepped on chain_stream,
-     * forked from and joined back into the engine stream around a run */
+     * forked from and joined back into the engine stream around a run;
+     * chain_me_a / chain_me_b: each chain's motion estimation done, the
+     * rungs that keep the chains half a step apart */
     hipStream_t chain_stream = nullptr;
     hipEvent_t chain_fork = nullptr, chain_join = nullptr;
+    hipEvent_t chain_me_a = nullptr, chain_me_b = nullptr;
     int chains = 0; /* 0 by size, 1 one chain, 2 two chains */
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
@@ -588,14 +591,19 @@ int enqueue_steps(h261g_engine *e, int k, bool timed)
         }
         return H261G_OK;
     }
-    /* the second chain starts after the first chain's first motion
-     * estimation: the halves then run about half a step apart, so one half's
-     * coder and writer overlap the other half's motion estimation */
+    /* A ladder: chain B's motion estimation of step i starts after chain
+     * A's, and chain A's of step i+1 after chain B's of step i.  The halves
+     * then run half a step apart, one half's coder and writer beside the
+     * other half's motion estimation.  Without the rungs nothing ties the
+     * chains together, and a captured run's launches reach the device one
+     * chain after the other: a chain would run most of its steps alone. */
+    if (!ok(hipEventRecord(e->chain_fork, st), "event") || !ok(hipStreamWaitEvent(e->chain_stream, e->chain_fork, 0), "wait"))
+        return H261G_ERR_DEVICE;
     for (int i = 0; i < k; i++) {
-        int rcode = enqueue_chain(e, st, 0, split, false, false, nullptr, i == 0 ? e->chain_fork : nullptr);
-        if (!rcode && i == 0 && !ok(hipStreamWaitEvent(e->chain_stream, e->chain_fork, 0), "wait"))
-            rcode = H261G_ERR_DEVICE;
-        if (!rcode) rcode = enqueue_chain(e, e->chain_stream, split, e->mb_used, false, false, nullptr);
+        if (i > 0 && !ok(hipStreamWaitEvent(st, e->chain_me_b, 0), "wait")) return H261G_ERR_DEVICE;
+        int rcode = enqueue_chain(e, st, 0, split, false, false, nullptr, e->chain_me_a);
+        if (!rcode && !ok(hipStreamWaitEvent(e->chain_stream, e->chain_me_a, 0), "wait")) rcode = H261G_ERR_DEVICE;
+        if (!rcode) rcode = enqueue_chain(e, e->chain_stream, split, e->mb_used, false, false, nullptr, e->chain_me_b);
         if (rcode) return rcode;
     }
     if (!ok(hipEventRecord(e->chain_join, e->chain_stream), "event") || !ok(hipStreamWaitEvent(st, e->chain_join, 0), "wait"))
@@ -738,6 +746,8 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipStreamCreateWithFlags(&e->chain_stream, hipStreamNonBlocking), "stream") &&
                 ok(hipEventCreateWithFlags(&e->chain_fork, hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->chain_join, hipEventDisableTiming), "event") &&
+                ok(hipEventCreateWithFlags(&e->chain_me_a, hipEventDisableTiming), "event") &&
+                ok(hipEventCreateWithFlags(&e->chain_me_b, hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->compute_ev, hipEventDisableTiming), "event") &&
                 ok(hipMalloc((void **)&e->desc, sizeof(StreamDesc) * max_streams), "hipMalloc") &&
                 ok(hipMalloc((void **)&e->seq, sizeof(SeqState) * max_streams), "hipMalloc") &&
@@ -806,6 +816,8 @@ int h261g_engine_destroy(h261g_engine_t *e)
     if (e->chain_stream) (void)hipStreamDestroy(e->chain_stream);
     if (e->chain_fork) (void)hipEventDestroy(e->chain_fork);
     if (e->chain_join) (void)hipEventDestroy(e->chain_join);
+    if (e->chain_me_a) (void)hipEventDestroy(e->chain_me_a);
+    if (e->chain_me_b) (void)hipEventDestroy(e->chain_me_b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return H261G_OK;

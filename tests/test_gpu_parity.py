@@ -247,6 +247,42 @@ def test_engine_batched_matches_oracle_and_alone(oracle):
     alone.destroy()
 
 
+def test_engine_first_stream_closed_keeps_others_exact(oracle):
+    """The stream holding MB 0 closes while the others go on (its range stays
+    unmapped, no re-packing): the others' bitstreams and reconstructions
+    still equal the oracle's, in steps and in graph replays.  Covers the
+    kernels' handling of lanes and slots with nothing to do, which must not
+    read through an unmapped MB's stream."""
+    specs = [(QCIF, 421), (CIF, 422), (QCIF, 423)]
+    n = 7
+    eng = Engine(max_streams=len(specs))
+    ids = []
+    for size, seed in specs:
+        sid = eng.open(*size, bitrate=160000, vbv_size=160000)
+        eng.generate_input(sid, seed, n, sigma=3, motion=15)
+        ids.append(sid)
+    got = {k: [] for k in range(1, len(specs))}
+    announce("first stream closed: 2 steps, close, 2 steps, graph replay of 3")
+    for t in range(n):
+        if t == 2:
+            eng.close(ids[0])
+        if t < 4:
+            eng.step()
+        elif t == 4:
+            eng.run(3)
+        eng.sync()
+        if t < 4 or t == n - 1:
+            for k in got:
+                got[k].append((t, eng.bitstream(ids[k])))
+    for k in got:
+        size, seed = specs[k]
+        s = oracle.stream(*size, bitrate=160000, vbv=160000)
+        want = [s.encode(*content_frame(seed, *size, t, sigma=3, motion=15))[0] for t in range(n)]
+        for t, bits in got[k]:
+            assert bits == want[t], f"stream {k} frame {t}"
+    eng.destroy()
+
+
 def test_engine_graph_run_matches_stepwise():
     """h261g_engine_run (HIP graph replay) codes the same frames as step()."""
     n = 6
