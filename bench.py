@@ -51,15 +51,16 @@ MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 3
 # HBM bytes per macroblock actually moved by each kernel with the XCD remap
 # of motion estimation and coder: FETCH_SIZE and WRITE_SIZE per dispatch from
 # two rocprofv3 --pmc passes over `bench.py --steps 8 --no-cpu --no-plain
-# --no-extra --copy-probe --shape 4,3,4` (the event pass's last 8 steps: all
-# 512 streams active, 126,720 MBs per dispatch; 4 MBs per coder wave and per
-# motion-estimation wave), each scaled by what the same counter reads for the
-# 256 MiB device copy of --copy-probe, divided by the dispatch's MB count
-# (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation
-# 73.7 MB read + 12.2 MB written, the coder 106.8 + 126.8 MB, the bitstream
-# writer (not remapped) 20.1 + 6.5 MB.  Other launch shapes: not measured.
-TRAFFIC_PER_MB = {("me_kernel", True): (73.71e6 + 12.15e6) / 126720,
-                  ("mb_kernel", True): (106.78e6 + 126.80e6) / 126720,
+# --no-extra --copy-probe --shape 4,3,4,2` (the event pass's last 8 steps: all
+# 512 streams active, 126,720 MBs per dispatch; 4 MBs per coder wave, runs of
+# 4 MBs per motion-estimation wave in search-window tiles of 2), each scaled
+# by what the same counter reads for the 256 MiB device copy of --copy-probe,
+# divided by the dispatch's MB count (profiles/batched_512_streams_hbm_traffic.md).
+# Motion estimation 74.2 MB read + 9.3 MB written, the coder 106.9 + 127.2 MB,
+# the bitstream writer (not remapped) 20.1 + 6.5 MB.  Other launch shapes: not
+# measured.
+TRAFFIC_PER_MB = {("me_kernel", True): (74.22e6 + 9.28e6) / 126720,
+                  ("mb_kernel", True): (106.87e6 + 127.21e6) / 126720,
                   ("bitstream_kernel", False): (20.10e6 + 6.47e6) / 126720}
 REMAP_BIT = {"me_kernel": 1, "mb_kernel": 2, "bitstream_kernel": 4}
 
