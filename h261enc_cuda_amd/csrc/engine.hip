@@ -165,11 +165,11 @@ struct h261g_engine {
     bool writer_pending = false;
     /* two chains: the second half of the MB space is stepped on chain_stream,
      * forked from and joined back into the engine stream around a run;
-     * chain_me_a / chain_me_b: each chain's motion estimation done, the
-     * rungs that keep the chains half a step apart */
+     * chain_me_a / chain_me_b[step & 1]: each chain's motion estimation
+     * done, the rungs that keep the chains together */
     hipStream_t chain_stream = nullptr;
     hipEvent_t chain_fork = nullptr, chain_join = nullptr;
-    hipEvent_t chain_me_a = nullptr, chain_me_b = nullptr;
+    hipEvent_t chain_me_a = nullptr, chain_me_b[2] = {nullptr, nullptr};
     int chains = 0; /* 0 by size, 1 one chain, 2 two chains */
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
@@ -591,19 +591,25 @@ int enqueue_steps(h261g_engine *e, int k, bool timed)
         }
         return H261G_OK;
     }
-    /* A ladder: chain B's motion estimation of step i starts after chain
-     * A's, and chain A's of step i+1 after chain B's of step i.  The halves
-     * then run half a step apart, one half's coder and writer beside the
-     * other half's motion estimation.  Without the rungs nothing ties the
-     * chains together, and a captured run's launches reach the device one
-     * chain after the other: a chain would run most of its steps alone. */
+    /* A ladder with one step of slack: chain B's motion estimation of step
+     * i starts after chain A's, and chain A's of step i+1 after chain B's of
+     * step i-1.  The halves then run about half a step apart, one half's
+     * coder and writer beside the other half's motion estimation, and A's
+     * next search may overlap B's current one.  Without the rungs nothing
+     * ties the chains together, and a captured run's launches reach the
+     * device one chain after the other: a chain would run most of its steps
+     * alone.  Without the slack the two searches never overlap, and once
+     * finished streams leave the halves small and uneven, a half's search
+     * alone does not fill the device (6% slower over the benchmark's job at
+     * 4 MBs per search wave). */
     if (!ok(hipEventRecord(e->chain_fork, st), "event") || !ok(hipStreamWaitEvent(e->chain_stream, e->chain_fork, 0), "wait"))
         return H261G_ERR_DEVICE;
     for (int i = 0; i < k; i++) {
-        if (i > 0 && !ok(hipStreamWaitEvent(st, e->chain_me_b, 0), "wait")) return H261G_ERR_DEVICE;
+        if (i > 1 && !ok(hipStreamWaitEvent(st, e->chain_me_b[i & 1], 0), "wait")) return H261G_ERR_DEVICE;
         int rcode = enqueue_chain(e, st, 0, split, false, false, nullptr, e->chain_me_a);
         if (!rcode && !ok(hipStreamWaitEvent(e->chain_stream, e->chain_me_a, 0), "wait")) rcode = H261G_ERR_DEVICE;
-        if (!rcode) rcode = enqueue_chain(e, e->chain_stream, split, e->mb_used, false, false, nullptr, e->chain_me_b);
+        if (!rcode)
+            rcode = enqueue_chain(e, e->chain_stream, split, e->mb_used, false, false, nullptr, e->chain_me_b[i & 1]);
         if (rcode) return rcode;
     }
     if (!ok(hipEventRecord(e->chain_join, e->chain_stream), "event") || !ok(hipStreamWaitEvent(st, e->chain_join, 0), "wait"))
@@ -747,7 +753,8 @@ int h261g_engine_create(h261g_engine_t **out, int32_t max_streams, int64_t hbm_b
                 ok(hipEventCreateWithFlags(&e->chain_fork, hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->chain_join, hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->chain_me_a, hipEventDisableTiming), "event") &&
-                ok(hipEventCreateWithFlags(&e->chain_me_b, hipEventDisableTiming), "event") &&
+                ok(hipEventCreateWithFlags(&e->chain_me_b[0], hipEventDisableTiming), "event") &&
+                ok(hipEventCreateWithFlags(&e->chain_me_b[1], hipEventDisableTiming), "event") &&
                 ok(hipEventCreateWithFlags(&e->compute_ev, hipEventDisableTiming), "event") &&
                 ok(hipMalloc((void **)&e->desc, sizeof(StreamDesc) * max_streams), "hipMalloc") &&
                 ok(hipMalloc((void **)&e->seq, sizeof(SeqState) * max_streams), "hipMalloc") &&
@@ -817,7 +824,8 @@ int h261g_engine_destroy(h261g_engine_t *e)
     if (e->chain_fork) (void)hipEventDestroy(e->chain_fork);
     if (e->chain_join) (void)hipEventDestroy(e->chain_join);
     if (e->chain_me_a) (void)hipEventDestroy(e->chain_me_a);
-    if (e->chain_me_b) (void)hipEventDestroy(e->chain_me_b);
+    for (hipEvent_t ev : e->chain_me_b)
+        if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return H261G_OK;
