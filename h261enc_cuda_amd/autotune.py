@@ -7,12 +7,12 @@ what it computes:
   (0 = chosen from the batch size).  More MBs per wave means fewer waves and
   fewer flushes of the denoise statistics, but longer waves.
 * `me_mbs_per_wave` -- consecutive macroblocks one motion-estimation wave
-  searches in turn (1, 2, 4): fewer, longer waves, one cost atomic per
-  stream segment of a run.
+  searches in turn (0 = chosen per launch from its size, 1, 2, 4): fewer,
+  longer waves, one cost atomic per stream segment of a run.
 * `me_tile` -- search-window tiling: macroblocks of a run that lie side by
   side in one row share one staged reference tile of up to 1, 2 or 4
-  windows (at most the run).  A wider tile fetches each reference pel fewer
-  times but takes more LDS per wave.
+  windows (at most the run; 0 = as wide as the run, up to 2).  A wider tile
+  fetches each reference pel fewer times but takes more LDS per wave.
 * `xcd_remap` -- bit 0 motion estimation, bit 1 coder, bit 2 bitstream
   writer: whether each XCD gets a contiguous run of macroblocks (neighbours
   share that XCD's L2) or workgroup i codes macroblock i.
@@ -22,9 +22,13 @@ mix of stream sizes, all streams active, inter frames -- with HIP events
 around every launch of the step, and keeps the fastest: first over
 `mbs_per_wave` at the default remap, then over the pairs (`me_mbs_per_wave`,
 `me_tile`), then over the remap at the best of both (coordinate descent; the
-knobs act on mostly different kernels).  Within `tie` of the fastest, the
-shortest motion-estimation run is preferred: the sample is at full load, and
-as streams finish, shorter runs keep the shrinking launches' waves numerous.
+knobs act on mostly different kernels).  The motion-estimation shape chosen
+per launch (0, 0) is kept unless a fixed shape beats it by more than twice
+`tie`: the sample is at full load, where the shapes differ by less than their
+noise (about 1.5%), while over a job whose streams finish the per-launch
+choice, which shortens the runs as launches shrink, is the fastest (the
+benchmark's whole job: 845k CIF fps by launch size, 840k at runs of 2, 822k at
+runs of 1, 812k at runs of 4).
 Among remaps within `tie` (1%) of the fastest, the fastest that remaps motion
 estimation is preferred: its time is the same within noise, and it reads
 about 7x fewer HBM bytes (neighbouring MBs' search windows meet in one XCD's
@@ -56,7 +60,7 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
     [{"mbs_per_wave", "me_mbs_per_wave", "me_tile", "xcd_remap", "step_ms",
     kernel times}]}.  A tile wider than the run is not tried."""
     mpw_c, remap_c, me_c, tile_c = list(mbs_per_wave), list(xcd_remap), list(me_mbs_per_wave), list(me_tile)
-    me_shapes = [(m, t) for m in me_c for t in tile_c if t <= max(m, 1)]
+    me_shapes = [(0, 0)] + [(m, t) for m in me_c for t in tile_c if t <= max(m, 1)]
     key = (tuple(sizes), tuple(mpw_c), tuple(remap_c), tuple(me_shapes), steps)
     if key in _CACHE:
         return _CACHE[key]
@@ -104,13 +108,9 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
         best_mpw = min((m for m in mpw_close if m), key=lambda m: mpw_t[m],
                        default=min(mpw_close, key=lambda m: mpw_t[m]))
         me_t = {sh: trial(best_mpw, default_remap, *sh) for sh in me_shapes}
-        # within the tie band the shortest run wins: the sample is at full
-        # load, where runs of 2 and 4 tie, but as streams finish the launches
-        # shrink and longer runs leave the device part empty (the benchmark's
-        # whole job: 838k CIF fps at runs of 2 against 812k at runs of 4)
-        me_fast = min(me_t.values())
-        me_close = [sh for sh in me_shapes if me_t[sh] <= me_fast * (1 + tie)]
-        best_me, best_tile = min(me_close, key=lambda sh: (sh[0], me_t[sh]))
+        # the per-launch shape stays unless a fixed one is clearly faster
+        fixed = min((sh for sh in me_shapes if sh != (0, 0)), key=lambda sh: me_t[sh], default=(0, 0))
+        best_me, best_tile = fixed if me_t[fixed] < me_t[(0, 0)] * (1 - 2 * tie) else (0, 0)
         times = {r: trial(best_mpw, r, best_me, best_tile) for r in remap_c}
         fastest = min(times.values())
         close = [r for r in remap_c if times[r] <= fastest * (1 + tie)]

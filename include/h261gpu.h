@@ -161,7 +161,7 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                 wave's run that lie side by side in one macroblock row share one
                                 staged reference tile of up to this many search windows: 1
                                 (every macroblock stages its own window), 2 or 4 (at most the
-                                run); 0 (default): chosen with the run */
+                                run); 0 (default): as wide as the run, at most 2 */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */
