@@ -218,7 +218,7 @@ def cpu_baseline(frames=30, runs=3, max_workers=16):
     cores, in the plain CPU configuration: one CIF stream of `frames` frames
     (fixed QUANT 8, global motion +-7, noise sigma 2) per worker process,
     each pinned to its own core of this process's affinity set (at most
-    max_workers: a GPU box's CPU share), distinct seeds, all started together
+    max_workers: the CPU share of a one-GPU machine), distinct seeds, all started together
     behind a barrier after generating their input.  Median of `runs` runs of
     aggregate frames/s (total frames / span from the first start to the last
     end) and of per-core frames/s."""
@@ -277,7 +277,7 @@ def main():
                          "autotuner's (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus above 1 is not supported "
                                    "(streams would shard across GPUs with no exchange)"}))
         sys.exit(2)
     from h261enc_cuda_amd import Engine, device_info

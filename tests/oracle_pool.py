@@ -65,7 +65,7 @@ def _ping() -> int:
 
 
 def pool_workers() -> int:
-    """at most 16 processes (a GPU box's CPU share)"""
+    """at most 16 processes (the CPU share of a one-GPU machine)"""
     return max(1, min(16, os.cpu_count() or 1))
 
 
