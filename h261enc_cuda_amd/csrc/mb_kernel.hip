@@ -454,8 +454,16 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             }
             __syncthreads();
 
-            /* ---- phase 2: lane (block, column c) ---- */
-            int blk_accum = 0;
+            /* ---- phase 2: lane (block, column c).  From the postscale
+             * rounding on, the coefficients stay floats: every value below
+             * is an integer of magnitude under 2^24, exact in FP32, so the
+             * denoise and the quantiser give the reference's integer results
+             * with FP32 instructions (twice the integer issue rate on this
+             * GPU): |c| - offset and the clamp at zero are exact, a float
+             * product's sign is the operands' ((float)(-a) * inv ==
+             * -(a * inv)), and truncation and the +-127 clamp commute with
+             * the sign ---- */
+            float blk_accum = 0.0f;
             if (L.act) {
                 const int c = rc;
                 float v[8];
@@ -465,50 +473,49 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                 float o8[8];
                 fdct_core(v[0] + v[7], v[1] + v[6], v[2] + v[5], v[3] + v[4], v[3] - v[4], v[2] - v[5],
                           v[1] - v[6], v[0] - v[7], o8);
-                int coef[8];
+                float cf[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++)
-                    coef[k] = (int)floorf((float)((double)(cs * kFwdPost[k] * o8[k]) + 0.49999));
+                for (int k = 0; k < 8; k++) cf[k] = floorf((float)((double)(cs * kFwdPost[k] * o8[k]) + 0.49999));
 
                 /* denoise with the previous frame's offsets, staged as UInt16
-                 * (mbgpu:525-528, 586-604) */
+                 * (mbgpu:525-528, 586-604); the statistics stay integer LDS
+                 * atomics (float ones are far slower when the 8 blocks of a
+                 * pass add to one address) */
 #pragma unroll
                 for (int r = 0; r < 8; r++) {
                     const int i = c + 8 * r;
-                    const int off = dn_off[L.slot][L.t][i];
-                    const int cc = coef[r];
-                    int av = cc < 0 ? -cc : cc;
-                    atomicAdd(&stat[L.slot][L.t][i], av);
-                    av = av - off;
-                    av = av > 0 ? av : 0;
-                    coef[r] = cc < 0 ? -av : av;
+                    const float off = (float)dn_off[L.slot][L.t][i];
+                    const float a = fabsf(cf[r]);
+                    atomicAdd(&stat[L.slot][L.t][i], (int)a);
+                    cf[r] = copysignf(fmaxf(a - off, 0.0f), cf[r]);
                 }
 
-                /* forward quantiser (mbgpu:685-716) */
+                /* forward quantiser (mbgpu:685-716): level = clamp(trunc(c *
+                 * inv), -127, 127), taken as sign(c) * min(trunc(|c| * inv), 127) */
                 const float inv = c_inv_quant[L.quant];
 #pragma unroll
                 for (int r = 0; r < 8; r++) {
                     const int i = c + 8 * r;
                     int l;
                     if (L.intra && i == 0) {
-                        l = coef[r] / 8;
+                        l = (int)cf[r] / 8;
                         l = clampi(l, 1, 254);
                         if (l == 128) l = 255;
-                        blk_accum += 100;
+                        blk_accum += 100.0f;
                     } else {
-                        const float f = (float)coef[r] * inv;
-                        l = clampi((int)f, -127, 127);
-                        const int al = l < 0 ? -l : l;
-                        blk_accum += al > 1 ? 100 : al;
+                        const float t = fminf(truncf(fabsf(cf[r]) * inv), 127.0f);
+                        l = (int)copysignf(t, cf[r]);
+                        blk_accum += t > 1.0f ? 100.0f : t;
                     }
                     lv[bi * 64 + i] = (int8_t)l;
                 }
             }
-            /* sum the 8 columns of the block */
-            blk_accum += __shfl_xor(blk_accum, 1, 64);
-            blk_accum += __shfl_xor(blk_accum, 2, 64);
-            blk_accum += __shfl_xor(blk_accum, 4, 64);
-            const bool coded = L.act && blk_accum > 2;
+            /* sum the 8 columns of the block: quads, then the two quads of
+             * the 8 lanes (row_half_mirror pairs lane j with 7 - j) */
+            blk_accum += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(blk_accum), 0xB1, 0xf, 0xf, false));
+            blk_accum += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(blk_accum), 0x4E, 0xf, 0xf, false));
+            blk_accum += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(blk_accum), 0x141, 0xf, 0xf, false));
+            const bool coded = L.act && blk_accum > 2.0f;
             /* coded blocks of the pass: ballot of the lanes c == 0 */
             const uint64_t coded_mask = __ballot(coded && rc == 0);
 #pragma unroll
