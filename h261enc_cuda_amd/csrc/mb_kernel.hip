@@ -394,57 +394,63 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     rw[k][1] = load_global<uint32_t>(q + a + 4);
                     rw[k][2] = load_global<uint32_t>(q + a2);
                 }
-                /* the 8 reference pels of rows rc-1, rc, rc+1 */
-                int a8[3][8];
+                /* MC, loop filter and residual on exact FP32 integers (pels,
+                 * filter sums up to 4 x 1020 + 8, residuals): FP32 issues at
+                 * twice the integer rate, the rounding shifts become products
+                 * by powers of two and floor, and v_cvt_f32_ubyte / v_cvt_pk_u8
+                 * move the pels between bytes and floats.  The 8 reference pels
+                 * of rows rc-1, rc, rc+1: */
+                float a8[3][8];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     const uint32_t lo = __builtin_amdgcn_alignbyte(rw[k][1], rw[k][0], o);
                     const uint32_t hi = __builtin_amdgcn_alignbyte(rw[k][2], rw[k][1], o);
 #pragma unroll
                     for (int x = 0; x < 4; x++) {
-                        a8[k][x] = (lo >> (8 * x)) & 0xff;
-                        a8[k][x + 4] = (hi >> (8 * x)) & 0xff;
+                        a8[k][x] = (float)((lo >> (8 * x)) & 0xffu);
+                        a8[k][x + 4] = (float)((hi >> (8 * x)) & 0xffu);
                     }
                 }
-                int pred[8];
+                float pred[8];
                 if (L.intra) {
 #pragma unroll
-                    for (int x = 0; x < 8; x++) pred[x] = 0;
+                    for (int x = 0; x < 8; x++) pred[x] = 0.0f;
                 } else if (!L.filter) {
 #pragma unroll
                     for (int x = 0; x < 8; x++) pred[x] = a8[1][x];
                 } else {
                     /* H.261 loop filter, separable [1 2 1]: 1-D at the block
-                     * edges, corners unfiltered (mbgpu:79-150) */
-                    int h[3][8];
+                     * edges, corners unfiltered (mbgpu:79-150).  Row weights
+                     * (1, 2, 1), or (0, 4, 0) on the block's first and last
+                     * rows: (4 h + 8) >> 4 == (h + 2) >> 2 */
+                    float h[3][8];
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
-                        h[k][0] = 4 * a8[k][0];
+                        h[k][0] = 4.0f * a8[k][0];
 #pragma unroll
-                        for (int x = 1; x < 7; x++) h[k][x] = a8[k][x - 1] + 2 * a8[k][x] + a8[k][x + 1];
-                        h[k][7] = 4 * a8[k][7];
+                        for (int x = 1; x < 7; x++) h[k][x] = __builtin_fmaf(2.0f, a8[k][x], a8[k][x - 1]) + a8[k][x + 1];
+                        h[k][7] = 4.0f * a8[k][7];
                     }
-                    if (rc == 0 || rc == 7) {
+                    const bool edge_row = rc == 0 || rc == 7;
+                    const float w02 = edge_row ? 0.0f : 1.0f, w1 = edge_row ? 4.0f : 2.0f;
 #pragma unroll
-                        for (int x = 0; x < 8; x++) pred[x] = (h[1][x] + 2) >> 2;
-                    } else {
-#pragma unroll
-                        for (int x = 0; x < 8; x++) pred[x] = (h[0][x] + 2 * h[1][x] + h[2][x] + 8) >> 4;
+                    for (int x = 0; x < 8; x++) {
+                        const float sum = __builtin_fmaf(w02, h[0][x] + h[2][x], __builtin_fmaf(w1, h[1][x], 8.0f));
+                        pred[x] = floorf(sum * 0.0625f);
                     }
                 }
-                int res[8];
+                float res[8];
 #pragma unroll
                 for (int x = 0; x < 4; x++) {
-                    res[x] = (int)((cw0 >> (8 * x)) & 0xff) - pred[x];
-                    res[x + 4] = (int)((cw1 >> (8 * x)) & 0xff) - pred[x + 4];
-                    predw[0] |= (uint32_t)pred[x] << (8 * x);
-                    predw[1] |= (uint32_t)pred[x + 4] << (8 * x);
+                    res[x] = (float)((cw0 >> (8 * x)) & 0xffu) - pred[x];
+                    res[x + 4] = (float)((cw1 >> (8 * x)) & 0xffu) - pred[x + 4];
+                    predw[0] = __builtin_amdgcn_cvt_pk_u8_f32(pred[x], (uint32_t)x, predw[0]);
+                    predw[1] = __builtin_amdgcn_cvt_pk_u8_f32(pred[x + 4], (uint32_t)x, predw[1]);
                 }
                 /* forward DCT, row pass */
                 float o8[8];
-                fdct_core((float)(res[0] + res[7]), (float)(res[1] + res[6]), (float)(res[2] + res[5]),
-                          (float)(res[3] + res[4]), (float)(res[3] - res[4]), (float)(res[2] - res[5]),
-                          (float)(res[1] - res[6]), (float)(res[0] - res[7]), o8);
+                fdct_core(res[0] + res[7], res[1] + res[6], res[2] + res[5], res[3] + res[4], res[3] - res[4],
+                          res[2] - res[5], res[1] - res[6], res[0] - res[7], o8);
 #pragma unroll
                 for (int k = 0; k < 8; k++) tf[k8 * kBlkPitch + rc * 8 + k] = o8[k];
             }
