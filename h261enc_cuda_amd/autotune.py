@@ -64,7 +64,7 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
     key = (tuple(sizes), tuple(mpw_c), tuple(remap_c), tuple(me_shapes), steps)
     if key in _CACHE:
         return _CACHE[key]
-    frames = 2 + (steps + 1) * (len(mpw_c) + len(remap_c) + len(me_shapes))
+    frames = 2 + (steps + 1) * (len(mpw_c) + len(remap_c) + 2 * len(me_shapes))
     eng = Engine(max_streams=len(sizes))
     # kernels alone (the writer at the end of its step, not beside the next
     # step's motion estimation), so the event spans add up to the step
@@ -107,7 +107,13 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
         mpw_close = [m for m in mpw_c if mpw_t[m] <= mpw_fast * (1 + tie)]
         best_mpw = min((m for m in mpw_close if m), key=lambda m: mpw_t[m],
                        default=min(mpw_close, key=lambda m: mpw_t[m]))
-        me_t = {sh: trial(best_mpw, default_remap, *sh) for sh in me_shapes}
+        # each shape twice, in forward then reverse order, averaged: the
+        # trials code successive frames of the sample, whose cost drifts,
+        # and the palindrome cancels a linear drift (the shapes differ by
+        # about as much as the drift over the sweep)
+        me_t = {sh: 0.0 for sh in me_shapes}
+        for sh in me_shapes + me_shapes[::-1]:
+            me_t[sh] += trial(best_mpw, default_remap, *sh) / 2
         # the per-launch shape stays unless a fixed one is clearly faster
         fixed = min((sh for sh in me_shapes if sh != (0, 0)), key=lambda sh: me_t[sh], default=(0, 0))
         best_me, best_tile = fixed if me_t[fixed] < me_t[(0, 0)] * (1 - 2 * tie) else (0, 0)
