@@ -51,24 +51,28 @@ MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 3
 # HBM bytes per macroblock actually moved by each kernel with the XCD remap
 # of motion estimation and coder: FETCH_SIZE and WRITE_SIZE per dispatch from
 # two rocprofv3 --pmc passes over `bench.py --steps 8 --no-cpu --no-plain
-# --no-extra --copy-probe --shape 4,3,4,2` (the event pass's last 8 steps: all
-# 512 streams active, 126,720 MBs per dispatch; 4 MBs per coder wave, runs of
-# 4 MBs per motion-estimation wave in search-window tiles of 2), each scaled
-# by what the same counter reads for the 256 MiB device copy of --copy-probe,
-# divided by the dispatch's MB count (profiles/batched_512_streams_hbm_traffic.md).
-# Motion estimation 74.2 MB read + 9.3 MB written, the coder 106.9 + 127.2 MB,
-# the bitstream writer (not remapped) 20.1 + 6.5 MB.  Other launch shapes: not
-# measured.
-TRAFFIC_PER_MB = {("me_kernel", True): (74.22e6 + 9.28e6) / 126720,
-                  ("mb_kernel", True): (106.87e6 + 127.21e6) / 126720,
-                  ("bitstream_kernel", False): (20.10e6 + 6.47e6) / 126720}
+# --no-extra --copy-probe --shape 4,3,R,T` (the event pass's last 8 steps: all
+# 512 streams active, 126,720 MBs per dispatch; 4 MBs per coder wave), each
+# scaled by what the same counter reads for the 256 MiB device copy of
+# --copy-probe, divided by the dispatch's MB count
+# (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation depends on
+# its run and tile (R, T): chosen per launch size (0, 0; runs of 2 MBs in one
+# tile at these launches) 80.3 MB read + 7.7 MB written, runs of 4 in tiles of
+# 2: 74.2 + 9.3 MB.  The coder 107.0 + 126.7 MB, the bitstream writer (not
+# remapped) 20.1 + 6.5 MB.  Other launch shapes: not measured.
+TRAFFIC_PER_MB = {("me_kernel", True, (0, 0)): (80.30e6 + 7.66e6) / 126720,
+                  ("me_kernel", True, (4, 2)): (74.22e6 + 9.28e6) / 126720,
+                  ("mb_kernel", True, None): (106.96e6 + 126.66e6) / 126720,
+                  ("bitstream_kernel", False, None): (20.10e6 + 6.47e6) / 126720}
 REMAP_BIT = {"me_kernel": 1, "mb_kernel": 2, "bitstream_kernel": 4}
 
 
-def traffic_per_launch(kernel, mbs_per_launch, xcd_remap):
+def traffic_per_launch(kernel, mbs_per_launch, tuned):
     """HBM traffic of an average launch: measured bytes per MB (for the
-    kernel's XCD remap setting) times the launch's MBs"""
-    t = TRAFFIC_PER_MB.get((kernel, bool(xcd_remap & REMAP_BIT[kernel])))
+    kernel's XCD remap setting and, for motion estimation, its run and tile)
+    times the launch's MBs"""
+    shape = (tuned["me_mbs_per_wave"], tuned["me_tile"]) if kernel == "me_kernel" else None
+    t = TRAFFIC_PER_MB.get((kernel, bool(tuned["xcd_remap"] & REMAP_BIT[kernel]), shape))
     return None if t is None else int(t * mbs_per_launch)
 
 
@@ -387,7 +391,7 @@ def main():
                 "unit": "T byte-SAD/s", "frac": round(achieved * 1e12 / SAD_PEAK, 4)}
     else:
         roof = {"kernel": dominant, "bound": "hbm", **hbm}
-    roof.update({"traffic": traffic_per_launch(dominant, job_k["mbs"], tuned["xcd_remap"]),
+    roof.update({"traffic": traffic_per_launch(dominant, job_k["mbs"], tuned),
                  "avg_launch_ms": round(job_k[key + "_ms"], 4),
                  "algorithmic_bytes_per_launch": int(job_k[key + "_bytes"]),
                  "algorithmic_byte_sads_per_launch": int(job_k["sad"]) if dominant == "me_kernel" else None,
@@ -398,11 +402,11 @@ def main():
         roof[name] = {"launches": f["launches"],
                       "mbs_per_launch": round(f["mbs"], 1),
                       "me_kernel": {"avg_launch_ms": round(f["me_ms"], 4), "hbm_gbs": round(f["me_gbs"], 1),
-                                    "traffic": traffic_per_launch("me_kernel", f["mbs"], tuned["xcd_remap"]),
+                                    "traffic": traffic_per_launch("me_kernel", f["mbs"], tuned),
                                     "sad_rate_tops": round(f["sad_rate"] / 1e12, 3),
                                     "sad_frac": round(f["sad_rate"] / SAD_PEAK, 4)},
                       "mb_kernel": {"avg_launch_ms": round(f["mb_ms"], 4), "hbm_gbs": round(f["mb_gbs"], 1),
-                                    "traffic": traffic_per_launch("mb_kernel", f["mbs"], tuned["xcd_remap"]),
+                                    "traffic": traffic_per_launch("mb_kernel", f["mbs"], tuned),
                                     "hbm_frac": round(f["mb_gbs"] / HBM_PEAK_GBS, 4)},
                       "frame_tail_ms_per_step": round(f["other_ms"], 4),
                       "bitstream_kernel_ms_per_step": round(f["bits_ms"], 4)}
