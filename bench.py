@@ -277,7 +277,7 @@ def main():
     ap.add_argument("--compact-every", type=int, default=30,
                     help="steps between closing finished streams and re-packing the MB space")
     ap.add_argument("--shape", default=None,
-                    help="MBS_PER_WAVE,XCD_REMAP[,ME_MBS_PER_WAVE[,ME_TILE]]: a fixed launch shape instead of the "
+                    help="MBS_PER_WAVE,XCD_REMAP[,ME_MBS_PER_WAVE[,ME_TILE[,ME_VARIANT]]]: a fixed launch shape instead of the "
                          "autotuner's (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -303,7 +303,7 @@ def main():
     if args.shape:
         v = [int(x) for x in args.shape.split(",")]
         tuned = {"mbs_per_wave": v[0], "xcd_remap": v[1], "me_mbs_per_wave": v[2] if len(v) > 2 else 0,
-                 "me_tile": v[3] if len(v) > 3 else 0, "fixed": True}
+                 "me_tile": v[3] if len(v) > 3 else 0, "me_variant": v[4] if len(v) > 4 else 0, "fixed": True}
     else:
         tuned = autotune.tune([s["size"] for s in specs])
 

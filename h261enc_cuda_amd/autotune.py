@@ -138,3 +138,4 @@ def apply(eng: Engine, tuned: dict) -> None:
     eng.config(nat.OPT_XCD_REMAP, tuned["xcd_remap"])
     eng.config(nat.OPT_ME_MBS_PER_WAVE, tuned.get("me_mbs_per_wave", 0))
     eng.config(nat.OPT_ME_TILE, tuned.get("me_tile", 0))
+    eng.config(nat.OPT_ME_VARIANT, tuned.get("me_variant", 0))

@@ -331,12 +331,33 @@ __device__ __forceinline__ void sad16(const uint32_t *win, const uint8_t *cur_mb
     uint64_t base = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(cm >> 32)) << 32) |
                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)cm);
     asm volatile("" : "+s"(base));
+    /* the rows come in batches of 4 (16 SGPRs), each waited for once: a
+     * scalar load shares the LDS wait counter and returns out of order, so
+     * while one is in flight every wait for a window read waits for all
+     * outstanding LDS reads too.  Inside a batch the window reads of
+     * successive rows then overlap (ME -2.5% in a same-box A/B) */
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const scalar_u32_ptr c = (scalar_u32_ptr)(uintptr_t)(base + (uint64_t)k * pitch);
-        const uint32_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    for (int kb = 0; kb < 16; kb += 4) {
+        uint32_t cr[4][4];
 #pragma unroll
-        for (int i = 0; i < NS; i++) acc[i] = sad_row(win + off[i] + k * kPitch, sh[i], c0, c1, c2, c3, acc[i]);
+        for (int k = 0; k < 4; k++) {
+            const scalar_u32_ptr c = (scalar_u32_ptr)(uintptr_t)(base + (uint64_t)(kb + k) * pitch);
+            cr[k][0] = c[0];
+            cr[k][1] = c[1];
+            cr[k][2] = c[2];
+            cr[k][3] = c[3];
+        }
+        asm volatile("" ::"s"(cr[0][0]), "s"(cr[0][1]), "s"(cr[0][2]), "s"(cr[0][3]), "s"(cr[1][0]), "s"(cr[1][1]),
+                     "s"(cr[1][2]), "s"(cr[1][3]));
+        asm volatile("" ::"s"(cr[2][0]), "s"(cr[2][1]), "s"(cr[2][2]), "s"(cr[2][3]), "s"(cr[3][0]), "s"(cr[3][1]),
+                     "s"(cr[3][2]), "s"(cr[3][3]));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int i = 0; i < NS; i++)
+                acc[i] = sad_row(win + off[i] + (kb + k) * kPitch, sh[i], cr[k][0], cr[k][1], cr[k][2], cr[k][3], acc[i]);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -522,6 +543,85 @@ __device__ __forceinline__ void stage_tile(const uint8_t *ref, int bx, int by, u
     static_assert(kIters <= 3 * kBatch, "three batches at most");
 }
 
+/* SATD costs (inter, inter + loop filter, intra), the reference's mode
+ * decision and the MB's result stores, once the vector (mvx, mvy) is found.
+ * Lane (4x4 sub-block, row y) holds its 4 current pels in crow and, for the
+ * prediction rows R - 1, R, R + 1 (R = the lane's MB row), pels -1..2 of its
+ * 4-pel group in lo[i] and pels 3, 4 in the low bytes of hi[i]. */
+__device__ __forceinline__ int me_finish(const MeLaunch &p, int g, const StreamDesc &d, uint32_t crow,
+                                         const uint32_t (&lo)[3], const uint32_t (&hi)[3], int R, int sub_x, int yq,
+                                         int mvx, int mvy, bool search)
+{
+    const int lane = threadIdx.x & 63;
+    /* SATD costs (inter, inter + loop filter, intra); 4 pels per lane as
+     * packed int16 pairs (p0, p2), (p1, p3) */
+    const short2v ce = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c020c00u));
+    const short2v co = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c030c01u));
+    const int yl = R & 7;
+    const bool row_border = yl == 0 || yl == 7;
+    /* [1 2 1] along rows R - 1, R, R + 1 of the prediction as packed pairs
+     * (h0, h2) and (h1, h3).  On a block's border rows the filtered values
+     * are discarded below, so every lane reads the same three rows and no
+     * lane branches */
+    uint32_t h02[3], h13[3], pm02 = 0, pm13 = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const uint32_t A = __builtin_amdgcn_perm(hi[i], lo[i], 0x0c020c00u); /* (p-1, p1) */
+        const uint32_t B = __builtin_amdgcn_perm(hi[i], lo[i], 0x0c030c01u); /* (p0, p2) */
+        const uint32_t C = __builtin_amdgcn_perm(hi[i], lo[i], 0x0c040c02u); /* (p1, p3) */
+        const uint32_t D = __builtin_amdgcn_perm(hi[i], lo[i], 0x0c050c03u); /* (p2, p4) */
+        h02[i] = A + 2 * B + C;
+        h13[i] = B + 2 * C + D;
+        if (i == 1) {
+            pm02 = B;
+            pm13 = C;
+        }
+    }
+    /* 3x3 [1 2 1] filter, rounded, on the 6x6 interior of each 8x8 block
+     * (megpu:833-879); pels on the block border keep the raw prediction */
+    const uint32_t f02 = (((h02[0] + 2 * h02[1] + h02[2]) + 0x00080008u) >> 4) & 0x00ff00ffu;
+    const uint32_t f13 = (((h13[0] + 2 * h13[1] + h13[2]) + 0x00080008u) >> 4) & 0x00ff00ffu;
+    const uint32_t m02 = row_border ? 0xffffffffu : ((sub_x & 1) ? 0u : 0x0000ffffu);
+    const uint32_t m13 = row_border ? 0xffffffffu : ((sub_x & 1) ? 0xffff0000u : 0u);
+    const uint32_t fp02 = (pm02 & m02) | (f02 & ~m02), fp13 = (pm13 & m13) | (f13 & ~m13);
+    const short2v s1 = (yq & 1) ? short2v{-1, -1} : short2v{1, 1};
+    const short2v s2 = (yq & 2) ? short2v{-1, -1} : short2v{1, 1};
+    const int t_inter = satd_quad(ce - as_s2(pm02), co - as_s2(pm13), s1, s2);
+    const int t_filter = satd_quad(ce - as_s2(fp02), co - as_s2(fp13), s1, s2);
+    const int t_intra = satd_quad(ce, co, s1, s2);
+    const int c_inter = search ? wave_sum(t_inter) : 0;
+    const int c_filter = search ? wave_sum(t_filter) : 0;
+    const int c_intra = wave_sum(t_intra);
+
+    /* h261_mode_decision (src/h261_moded.c:31-100), on wave-uniform values */
+    const bool md_intra = d.frame_type == H261_FRAME_TYPE_INTRA || (c_intra <= c_inter && c_intra <= c_filter);
+    const bool md_filter = !md_intra && c_filter < c_inter;
+    int md_flags = md_filter ? (H261_MB_MC | H261_MB_FILTER) : 0;
+    if (!md_intra && (mvx != 0 || mvy != 0)) md_flags |= H261_MB_MC;
+    const int md_cost = md_intra ? c_intra : (md_filter ? c_filter : c_inter);
+
+    /* results: MeResult (5 dwords) from lanes 0-4, MbMode (6 dwords) from
+     * lanes 8-13 */
+    if (lane < 5) {
+        const uint32_t mv = (uint32_t)(uint8_t)(int8_t)mvx | ((uint32_t)(uint8_t)(int8_t)mvy << 8);
+        const uint32_t w = lane == 0 ? (uint32_t)c_intra : lane == 1 ? 0u : lane == 2 ? (uint32_t)c_inter
+                         : lane == 3 ? (uint32_t)c_filter : mv;
+        ((uint32_t *)(p.result + g))[lane] = w;
+    }
+    if (p.mode && lane >= 8 && lane < 14) {
+        /* MbMode {type, flags, mvx, mvy, quant_adjust, cost} */
+        const int k = lane - 8;
+        const int w = k == 0 ? (md_intra ? H261_MB_TYPE_INTRA : H261_MB_TYPE_INTER) : k == 1 ? md_flags
+                    : k == 2 ? (md_intra ? 0 : mvx) : k == 3 ? (md_intra ? 0 : mvy) : k == 4 ? 0 : md_cost;
+        ((int32_t *)(p.mode + g))[k] = w;
+    }
+    if (lane == 0 && p.mv_start) {
+        p.mv_start[2 * g] = (int8_t)mvx;
+        p.mv_start[2 * g + 1] = (int8_t)mvy;
+    }
+    return md_cost;
+}
+
 /* One macroblock of a W-pel-wide frame (352 CIF, 176 QCIF: every size and
  * divisor below is a compile-time constant, so MB positions, plane offsets
  * and the current MB's scalar row loads need no run-time arithmetic).  The
@@ -635,81 +735,20 @@ __device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc 
         mvy = sy + wty - 3;
     }
 
-    /* SATD costs (inter, inter + loop filter, intra); 4 pels per lane as
-     * packed int16 pairs (p0, p2), (p1, p3) */
-    const short2v ce = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c020c00u));
-    const short2v co = as_s2(__builtin_amdgcn_perm(crow, crow, 0x0c030c01u));
-    /* window: row of the predicted MB row, byte of pel -1 of this lane */
+    /* SATD rows: window row of the predicted MB row, byte of pel -1 of
+     * this lane; rows -1 and 46 are spare window rows */
     const int wr = mvy + 15 + R;
     const int b0 = mvx + 16 + 4 * sub_x - 1;
     const int q = b0 >> 2, bsh = b0 & 3, q2 = min(q + 2, kWinDw - 1);
-    const int yl = R & 7;
-    const bool row_border = yl == 0 || yl == 7;
-    /* [1 2 1] along rows wr - 1, wr, wr + 1 as packed pairs (h0, h2) and
-     * (h1, h3).  Rows -1 and 46 are spare window rows; on a block's border
-     * rows the filtered values are discarded below, so every lane reads the
-     * same three rows and no lane branches */
-    uint32_t h02[3], h13[3], pm02 = 0, pm13 = 0;
+    uint32_t lo[3], hi[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const uint32_t *w = win + (wr - 1 + i) * kPitch;
         const uint32_t w0 = w[q], w1 = w[q + 1], w2 = w[q2];
-        const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, bsh); /* pels -1, 0, 1, 2 */
-        const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, bsh); /* pels 3, 4 */
-        const uint32_t A = __builtin_amdgcn_perm(hi, lo, 0x0c020c00u); /* (p-1, p1) */
-        const uint32_t B = __builtin_amdgcn_perm(hi, lo, 0x0c030c01u); /* (p0, p2) */
-        const uint32_t C = __builtin_amdgcn_perm(hi, lo, 0x0c040c02u); /* (p1, p3) */
-        const uint32_t D = __builtin_amdgcn_perm(hi, lo, 0x0c050c03u); /* (p2, p4) */
-        h02[i] = A + 2 * B + C;
-        h13[i] = B + 2 * C + D;
-        if (i == 1) {
-            pm02 = B;
-            pm13 = C;
-        }
+        lo[i] = __builtin_amdgcn_alignbyte(w1, w0, bsh); /* pels -1, 0, 1, 2 */
+        hi[i] = __builtin_amdgcn_alignbyte(w2, w1, bsh); /* pels 3, 4 */
     }
-    /* 3x3 [1 2 1] filter, rounded, on the 6x6 interior of each 8x8 block
-     * (megpu:833-879); pels on the block border keep the raw prediction */
-    const uint32_t f02 = (((h02[0] + 2 * h02[1] + h02[2]) + 0x00080008u) >> 4) & 0x00ff00ffu;
-    const uint32_t f13 = (((h13[0] + 2 * h13[1] + h13[2]) + 0x00080008u) >> 4) & 0x00ff00ffu;
-    const uint32_t m02 = row_border ? 0xffffffffu : ((sub_x & 1) ? 0u : 0x0000ffffu);
-    const uint32_t m13 = row_border ? 0xffffffffu : ((sub_x & 1) ? 0xffff0000u : 0u);
-    const uint32_t fp02 = (pm02 & m02) | (f02 & ~m02), fp13 = (pm13 & m13) | (f13 & ~m13);
-    const short2v s1 = (yq & 1) ? short2v{-1, -1} : short2v{1, 1};
-    const short2v s2 = (yq & 2) ? short2v{-1, -1} : short2v{1, 1};
-    const int t_inter = satd_quad(ce - as_s2(pm02), co - as_s2(pm13), s1, s2);
-    const int t_filter = satd_quad(ce - as_s2(fp02), co - as_s2(fp13), s1, s2);
-    const int t_intra = satd_quad(ce, co, s1, s2);
-    const int c_inter = search ? wave_sum(t_inter) : 0;
-    const int c_filter = search ? wave_sum(t_filter) : 0;
-    const int c_intra = wave_sum(t_intra);
-
-    /* h261_mode_decision (src/h261_moded.c:31-100), on wave-uniform values */
-    const bool md_intra = d.frame_type == H261_FRAME_TYPE_INTRA || (c_intra <= c_inter && c_intra <= c_filter);
-    const bool md_filter = !md_intra && c_filter < c_inter;
-    int md_flags = md_filter ? (H261_MB_MC | H261_MB_FILTER) : 0;
-    if (!md_intra && (mvx != 0 || mvy != 0)) md_flags |= H261_MB_MC;
-    const int md_cost = md_intra ? c_intra : (md_filter ? c_filter : c_inter);
-
-    /* results: MeResult (5 dwords) from lanes 0-4, MbMode (6 dwords) from
-     * lanes 8-13 */
-    if (lane < 5) {
-        const uint32_t mv = (uint32_t)(uint8_t)(int8_t)mvx | ((uint32_t)(uint8_t)(int8_t)mvy << 8);
-        const uint32_t w = lane == 0 ? (uint32_t)c_intra : lane == 1 ? 0u : lane == 2 ? (uint32_t)c_inter
-                         : lane == 3 ? (uint32_t)c_filter : mv;
-        ((uint32_t *)(p.result + g))[lane] = w;
-    }
-    if (p.mode && lane >= 8 && lane < 14) {
-        /* MbMode {type, flags, mvx, mvy, quant_adjust, cost} */
-        const int k = lane - 8;
-        const int w = k == 0 ? (md_intra ? H261_MB_TYPE_INTRA : H261_MB_TYPE_INTER) : k == 1 ? md_flags
-                    : k == 2 ? (md_intra ? 0 : mvx) : k == 3 ? (md_intra ? 0 : mvy) : k == 4 ? 0 : md_cost;
-        ((int32_t *)(p.mode + g))[k] = w;
-    }
-    if (lane == 0 && p.mv_start) {
-        p.mv_start[2 * g] = (int8_t)mvx;
-        p.mv_start[2 * g + 1] = (int8_t)mvy;
-    }
-    return md_cost;
+    return me_finish(p, g, d, crow, lo, hi, R, sub_x, yq, mvx, mvy, search);
 }
 
 /* The MB costs a wave found for one stream enter its rate control with one
@@ -719,7 +758,7 @@ __device__ __forceinline__ int me_mb(const MeLaunch &p, int g, const StreamDesc 
  * accumulate for the rate-control begin launch. */
 __device__ __forceinline__ void add_costs(const MeLaunch &p, int s, const StreamDesc &d, int count, uint32_t cost)
 {
-    if (threadIdx.x != 0 || count == 0 || !p.mode || !p.rc) return;
+    if ((threadIdx.x & 63) != 0 || count == 0 || !p.mode || !p.rc) return;
     RcState &rc = p.rc[s];
     if (p.fuse_tail) {
         const unsigned long long old = atomicAdd((unsigned long long *)&rc.me_acc, ((unsigned long long)count << 32) | cost);
@@ -738,7 +777,9 @@ __device__ __forceinline__ void add_costs(const MeLaunch &p, int s, const Stream
  * one stream add their costs with one atomic, and up to kTile MBs of a run
  * that lie side by side in one MB row search one staged tile. */
 template <int kVariant, int kMbs, int kTile>
-__global__ __launch_bounds__(64, 7) void me_kernel(MeLaunch p)
+/* 8 waves per SIMD (64 VGPRs; 11-19 kernel arguments and descriptor fields
+ * then live in VGPR lanes, about 15 v_readlane per MB): 2% faster than 7 */
+__global__ __launch_bounds__(64, 8) void me_kernel(MeLaunch p)
 {
     static_assert(kTile == 1 || (kVariant == kMeTwoStage && kTile <= kMbs), "tiles: two-stage search, within a run");
     __shared__ __align__(16) uint32_t winmem[win_alloc<kVariant, kTile>()];
