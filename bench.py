@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the hard-motion and serving lines")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--chain-ladder", type=int, default=0, choices=(0, 1, 2),
+                    help="two-chain ladder (H261G_OPT_CHAIN_LADDER): 0 by size, 1 strict, 2 one step of slack")
     ap.add_argument("--compact-every", type=int, default=30,
                     help="steps between closing finished streams and re-packing the MB space")
     ap.add_argument("--shape", default=None,
@@ -285,6 +287,7 @@ def main():
                                    "(streams would shard across GPUs with no exchange)"}))
         sys.exit(2)
     from h261enc_cuda_amd import Engine, device_info
+    from h261enc_cuda_amd import _native as nat
     dev = device_info()
 
     if args.copy_probe:
@@ -309,6 +312,7 @@ def main():
 
     eng = Engine(max_streams=len(specs))
     autotune.apply(eng, tuned)
+    eng.config(nat.OPT_CHAIN_LADDER, args.chain_ladder)
     ids = open_streams(eng, specs)
     for _ in range(W - 1):
         eng.step()
@@ -329,9 +333,9 @@ def main():
     # Each kernel runs alone here: the bitstream writer at the end of its own
     # step rather than beside the next step's motion estimation, so no event
     # span covers two kernels
-    from h261enc_cuda_amd import _native as nat
     eng = Engine(max_streams=len(specs))
     autotune.apply(eng, tuned)
+    eng.config(nat.OPT_CHAIN_LADDER, args.chain_ladder)
     eng.config(nat.OPT_OVERLAP_WRITER, 0)
     ids = open_streams(eng, specs)
     for _ in range(W):
@@ -354,6 +358,7 @@ def main():
     eng = Engine(max_streams=len(specs))
     eng.config(nat.OPT_DEVICE_BITSTREAM, 0)
     autotune.apply(eng, tuned)
+    eng.config(nat.OPT_CHAIN_LADDER, args.chain_ladder)
     ids = open_streams(eng, specs)
     for _ in range(W - 1):
         eng.step()

@@ -67,6 +67,7 @@ OPT_OVERLAP_WRITER = 7
 OPT_CHAINS = 8
 OPT_ME_MBS_PER_WAVE = 9
 OPT_ME_TILE = 10
+OPT_CHAIN_LADDER = 11
 
 
 class StreamParams(ctypes.Structure):

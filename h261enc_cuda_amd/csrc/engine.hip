@@ -171,6 +171,7 @@ struct h261g_engine {
     hipEvent_t chain_fork = nullptr, chain_join = nullptr;
     hipEvent_t chain_me_a = nullptr, chain_me_b[2] = {nullptr, nullptr};
     int chains = 0; /* 0 by size, 1 one chain, 2 two chains */
+    int ladder = 0; /* two chains: 0 by size, 1 strict, 2 one step of slack */
     int async_input = 0;
     int mbs_per_wave = 0; /* 0: chosen per launch from the MB count */
     int me_mbs_per_wave = 0; /* motion estimation: 0 chosen per launch, else 1, 2, 4 or 8 */
@@ -427,7 +428,7 @@ int read_oldest_times(h261g_engine *e);
 constexpr int kFillWaves = 256 * 4 * 5;
 /* motion-estimation waves resident on the device (256 CUs x 4 SIMDs x 7) and
  * the MBs per wave of a launch that holds several times as many MBs */
-constexpr int kMeResidentWaves = 256 * 4 * 7;
+constexpr int kMeResidentWaves = 256 * 4 * 8;
 constexpr int kMeRunDefault = 2;
 constexpr int kMeTileDefault = 2;
 
@@ -591,21 +592,30 @@ int enqueue_steps(h261g_engine *e, int k, bool timed)
         }
         return H261G_OK;
     }
-    /* A ladder with one step of slack: chain B's motion estimation of step
-     * i starts after chain A's, and chain A's of step i+1 after chain B's of
-     * step i-1.  The halves then run about half a step apart, one half's
-     * coder and writer beside the other half's motion estimation, and A's
-     * next search may overlap B's current one.  Without the rungs nothing
-     * ties the chains together, and a captured run's launches reach the
-     * device one chain after the other: a chain would run most of its steps
-     * alone.  Without the slack the two searches never overlap, and once
-     * finished streams leave the halves small and uneven, a half's search
-     * alone does not fill the device (6% slower over the benchmark's job at
-     * 4 MBs per search wave). */
+    /* A ladder of motion-estimation events ties the chains: chain B's
+     * search of step i starts after chain A's.  Without the rungs nothing
+     * ties them, and a captured run's launches reach the device one chain
+     * after the other: a chain would run most of its steps alone.
+     *   strict: chain A's search of step i+1 starts after chain B's of step
+     *   i.  The searches alternate, so each half's coder, frame tail and
+     *   writer run beside the other half's search, and the latency-bound
+     *   tails never meet on an otherwise idle device.
+     *   one step of slack: chain A's search of step i+1 starts after chain
+     *   B's of step i-1, so A's next search may overlap B's current one.
+     *   Once the halves run at the same pace, B trails A by a whole step and
+     *   both halves' tails coincide, but when finished streams leave the
+     *   halves small and uneven, a half's search alone does not fill the
+     *   device and the strict form loses (6% over the benchmark's job at 4
+     *   MBs per search wave).
+     * By size: strict while each half holds four times the search waves the
+     * device keeps resident. */
+    const int half = split < e->mb_used - split ? split : e->mb_used - split;
+    const bool strict = e->ladder == 1 || (e->ladder == 0 && half >= 4 * kMeResidentWaves);
     if (!ok(hipEventRecord(e->chain_fork, st), "event") || !ok(hipStreamWaitEvent(e->chain_stream, e->chain_fork, 0), "wait"))
         return H261G_ERR_DEVICE;
     for (int i = 0; i < k; i++) {
-        if (i > 1 && !ok(hipStreamWaitEvent(st, e->chain_me_b[i & 1], 0), "wait")) return H261G_ERR_DEVICE;
+        const int rung = strict ? i - 1 : i - 2; /* B's search that A's of step i waits for */
+        if (rung >= 0 && !ok(hipStreamWaitEvent(st, e->chain_me_b[rung & 1], 0), "wait")) return H261G_ERR_DEVICE;
         int rcode = enqueue_chain(e, st, 0, split, false, false, nullptr, e->chain_me_a);
         if (!rcode && !ok(hipStreamWaitEvent(e->chain_stream, e->chain_me_a, 0), "wait")) rcode = H261G_ERR_DEVICE;
         if (!rcode)
@@ -687,6 +697,10 @@ int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value)
     case H261G_OPT_CHAINS:
         if (value < 0 || value > 2) return H261G_ERR_ARG;
         e->chains = value;
+        break;
+    case H261G_OPT_CHAIN_LADDER:
+        if (value < 0 || value > 2) return H261G_ERR_ARG;
+        e->ladder = value;
         break;
     case H261G_OPT_ME_MBS_PER_WAVE:
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return H261G_ERR_ARG;

@@ -162,6 +162,12 @@ H261G_API int h261g_engine_destroy(h261g_engine_t *e);
                                 staged reference tile of up to this many search windows: 1
                                 (every macroblock stages its own window), 2 or 4 (at most the
                                 run); 0 (default): as wide as the run, at most 2 */
+#define H261G_OPT_CHAIN_LADDER 11 /* two chains (H261G_OPT_CHAINS): 1: a half's motion estimation of step
+                                     i+1 starts after the other half's of step i, so the halves stay half
+                                     a step apart (one half's coder, frame tail and writer beside the
+                                     other's search); 2: one step of slack (a half's search may overlap
+                                     the other's); 0 (default): 1 while each half holds enough
+                                     macroblocks to fill the device with its search alone, else 2 */
 H261G_API int h261g_engine_config(h261g_engine_t *e, int32_t option, int32_t value);
 
 /* the device the library runs on (the current HIP device) */

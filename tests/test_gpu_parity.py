@@ -312,17 +312,20 @@ def test_engine_graph_run_matches_stepwise():
 
 def test_engine_two_chains_match_one_chain(oracle):
     """Two chains (the MB space split at a stream boundary, each half stepped
-    on its own HIP stream) code every stream as one chain does, stepwise and
+    on its own HIP stream), tied by the strict ladder or by the ladder with
+    one step of slack, code every stream as one chain does, stepwise and
     under graph replay, across a close and a re-packing; one stream is also
     checked against the oracle."""
-    from h261enc_cuda_amd._native import OPT_CHAINS, OPT_FUSED_TAILS
+    from h261enc_cuda_amd._native import OPT_CHAIN_LADDER, OPT_CHAINS, OPT_FUSED_TAILS
     specs = [(CIF, 301, 0, 256000), (QCIF, 302, 0, 64000), (QCIF, 303, 6, 0), (CIF, 304, 0, 384000),
              (QCIF, 305, 0, 128000)]
     n = 10
     engines = []
-    for chains in (1, 2):
+    # one chain; two chains with the strict ladder and with one step of slack
+    for chains, ladder in ((1, 0), (2, 1), (2, 2)):
         eng = Engine(max_streams=8)
         eng.config(OPT_CHAINS, chains)
+        eng.config(OPT_CHAIN_LADDER, ladder)
         if chains == 2:  # each half's rate-control begin as its own launch, as in full-size steps
             eng.config(OPT_FUSED_TAILS, 2)
         sids = []
@@ -334,15 +337,16 @@ def test_engine_two_chains_match_one_chain(oracle):
     announce("engine two chains vs one chain")
 
     def compare(what):
-        (a, ia), (b, ib) = engines
-        for k, (sa, sb) in enumerate(zip(ia, ib)):
-            if sa is None:
-                continue
-            assert a.bitstream(sa) == b.bitstream(sb), f"{what}: stream {k} bitstream"
-            assert a.info(sa).quant == b.info(sb).quant, f"{what}: stream {k} quantiser"
-            ra, rb = a.recon(sa), b.recon(sb)
-            for p in range(3):
-                assert np.array_equal(ra[p], rb[p]), f"{what}: stream {k} recon"
+        a, ia = engines[0]
+        for j, (b, ib) in enumerate(engines[1:], 1):
+            for k, (sa, sb) in enumerate(zip(ia, ib)):
+                if sa is None:
+                    continue
+                assert a.bitstream(sa) == b.bitstream(sb), f"{what}: engine {j} stream {k} bitstream"
+                assert a.info(sa).quant == b.info(sb).quant, f"{what}: engine {j} stream {k} quantiser"
+                ra, rb = a.recon(sa), b.recon(sb)
+                for p in range(3):
+                    assert np.array_equal(ra[p], rb[p]), f"{what}: engine {j} stream {k} recon"
 
     size, seed, fixed, rate = specs[3]
     os_ = oracle.stream(*size, fixed_quant=fixed, bitrate=rate, vbv=rate)
@@ -352,7 +356,8 @@ def test_engine_two_chains_match_one_chain(oracle):
             eng.sync()
         compare(f"step {t}")
         ob, _, _, _, _ = os_.encode(*content_frame(seed, *size, t, sigma=2, motion=15))
-        assert engines[1][0].bitstream(engines[1][1][3]) == ob, f"two chains vs oracle, frame {t}"
+        for eng, sids in engines[1:]:
+            assert eng.bitstream(sids[3]) == ob, f"two chains vs oracle, frame {t}"
     for eng, _ in engines:
         eng.run(5)
         eng.sync()
