@@ -179,6 +179,18 @@ __device__ __forceinline__ void idct_core(const float *t, float *o)
     o[7] = tmp0 - tmp13;
 }
 
+/* sum over the wave without LDS round trips: quads, half rows and rows by
+ * DPP, the four rows by readlane */
+__device__ __forceinline__ int wave_sum_i32(int x)
+{
+    x += __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false);  /* quad_perm [1,0,3,2] */
+    x += __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false);  /* quad_perm [2,3,0,1] */
+    x += __builtin_amdgcn_mov_dpp(x, 0x141, 0xf, 0xf, false); /* row_half_mirror */
+    x += __builtin_amdgcn_mov_dpp(x, 0x140, 0xf, 0xf, false); /* row_mirror */
+    return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) + __builtin_amdgcn_readlane(x, 32) +
+           __builtin_amdgcn_readlane(x, 48);
+}
+
 /* bits of one TCOEFF event: the code and its sign bit, or an escape; tlen
  * is the wave's LDS copy of the code lengths plus one (kEscapeBits where the
  * table has no code) */
@@ -633,11 +645,17 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 
             int bits = 0;
             if (p.side && (intra || cbp)) {
-                /* lane k = transmission position k of each block in turn */
+                /* lane k = transmission position k of each block in turn; the
+                 * written blocks' levels are read up front */
+                int lk[6];
+#pragma unroll
+                for (int k = 0; k < 6; k++)
+                    lk[k] = (intra || (cbp & kCbpBit[k])) ? (int)mlv[k * 64 + zz] : 0;
+#pragma unroll
                 for (int k = 0; k < 6; k++) {
                     const bool blk_written = intra || (cbp & kCbpBit[k]);
                     if (!blk_written) continue;
-                    const int l = mlv[k * 64 + zz]; /* an intra DC reads as non-zero either way */
+                    const int l = lk[k]; /* an intra DC reads as non-zero either way */
                     const uint64_t nz = __ballot(l != 0);
                     if (l != 0) {
                         const uint64_t below = lane ? (nz & ((~0ull) >> (64 - lane))) : 0ull;
@@ -656,8 +674,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     }
                     if (lane == 0) bits += 2; /* EOB */
                 }
-#pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) bits += __shfl_xor(bits, off, 64);
+                bits = wave_sum_i32(bits);
                 if (!intra) bits += c_cbp[cbp].len;
             }
 
