@@ -33,7 +33,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from h261enc_cuda_amd.workloads import batched_streams, hard_streams, open_streams, run_job  # noqa: E402
+from h261enc_cuda_amd.workloads import batched_streams, content_kwargs, hard_streams, open_streams, run_job  # noqa: E402
 
 METRIC = "CIF frames/s on 1 MI355X, batched vs plain, bit-exact; p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -183,6 +183,54 @@ def hard_motion(Engine, steps=120, mbs_per_wave=0):
     eng.destroy()
     return {"streams": len(specs), "steps": steps - 2, "frames": frames, "seconds": round(dt, 3),
             "frames_per_s": round(frames / dt, 1), "cif_frames_per_s": round(mbs / 396.0 / dt, 1)}
+
+
+def host_fed(Engine, steps=20, warmup=3):
+    """The batched streams with their input in host memory: each step uploads
+    every stream's frame from one pinned host buffer (h261g_engine_put_frames:
+    one copy across PCIe and one kernel that places the frames, nothing
+    waits), codes it, and fetches every bitstream back to host memory
+    (h261g_engine_fetch_bitstreams, collected one step later).  PCIe is
+    inside the timed region.  Two frames per stream alternate (the search
+    cost does not depend on the content); all 512 streams are active in every
+    step."""
+    import torch
+    from h261enc_cuda_amd import content_frame
+    specs = batched_streams()
+    eng = Engine(max_streams=len(specs))
+    ids = [eng.open(*s["size"], bitrate=s["bitrate"], vbv_size=s["bitrate"], init_quant=10) for s in specs]
+    sets = []
+    for t in range(2):
+        buf, offs = Engine.pack_frames([content_frame(s["seed"], *s["size"], t, **content_kwargs(s)) for s in specs])
+        pin = torch.empty(buf.size, dtype=torch.uint8, pin_memory=True)
+        pbuf = pin.numpy()
+        pbuf[...] = buf
+        sets.append((pin, pbuf, offs))
+    up_bytes = sets[0][1].size
+
+    def one_step(t):
+        _, pbuf, offs = sets[t % 2]
+        eng.put_frames(ids, pbuf, offs)
+        eng.run(1)
+        eng.fetch_bitstreams(ids)  # waits for the previous step's fetch
+
+    for t in range(warmup):
+        one_step(t)
+    eng.sync()
+    eng.fetch_wait()
+    t0 = time.perf_counter()
+    for t in range(warmup, warmup + steps):
+        one_step(t)
+    down = eng.fetch_wait()
+    dt = time.perf_counter() - t0
+    eng.destroy()
+    cif = steps * sum((s["size"][0] // 16) * (s["size"][1] // 16) for s in specs) / 396.0
+    return {"streams": len(specs), "steps": steps, "seconds": round(dt, 3),
+            "cif_frames_per_s": round(cif / dt, 1), "ms_per_step": round(dt * 1e3 / steps, 4),
+            "upload_mb_per_step": round(up_bytes / 1e6, 2),
+            "upload_gb_s": round(up_bytes * steps / dt / 1e9, 1),
+            "last_step_bitstream_kb": round(down / 1e3, 1),
+            "input": "one pinned host buffer per step (put_frames), every bitstream fetched to host memory"}
 
 
 def serving(rate=2000.0, duration=0.5):
@@ -421,10 +469,11 @@ def main():
         fps, p50, p99 = plain_single_stream(Engine)
         plain = {"fps": round(fps, 1), "latency_ms_p50": round(p50, 3), "latency_ms_p99": round(p99, 3)}
 
-    hard = serve = None
+    hard = serve = fed = None
     if not args.no_extra:
         hard = hard_motion(Engine)
         serve = serving()
+        fed = host_fed(Engine)
 
     cpu = None
     if not args.no_cpu:
@@ -448,6 +497,7 @@ def main():
                    "plain_single_cif": plain,
                    "batched_over_plain": round(value / plain["fps"], 2) if plain else None,
                    "hard_motion_batch": hard, "serving": serve,
+                   "host_fed_batch": fed,
                    "parallelism": "1 GPU, all streams per frame step",
                    "autotune": tuned,
                    "device": f"{dev['name']} {dev['arch']}, {dev['compute_units']} CUs, {dev['clock_mhz']} MHz"},

@@ -54,7 +54,7 @@ EXPORTS = [
     "h261g_stream_read_me", "h261g_stream_read_bitstream", "h261g_stream_read_input",
     "h261g_engine_profile", "h261g_engine_kernel_times", "h261g_content_frame_host", "h261g_write_picture",
     "h261g_engine_config", "h261g_device_info", "h261g_stream_input_pending", "h261g_engine_fetch_bitstreams",
-    "h261g_engine_fetched", "h261g_engine_fetch_bitstreams_into",
+    "h261g_engine_fetched", "h261g_engine_fetch_bitstreams_into", "h261g_engine_put_frames",
 ]
 FETCH_INTO_MAX_STREAMS = 4096  # H261G_FETCH_INTO_MAX_STREAMS
 OPT_ME_VARIANT = 1
@@ -117,6 +117,7 @@ def load() -> ctypes.CDLL:
         "h261g_stream_open": (ctypes.c_int, [P, ctypes.POINTER(StreamParams)]),
         "h261g_stream_close": (ctypes.c_int, [P, i32]),
         "h261g_stream_put_frame": (ctypes.c_int, [P, i32, P, P, P]),
+        "h261g_engine_put_frames": (ctypes.c_int, [P, i32, P, P, i64, P]),
         "h261g_stream_generate_input": (ctypes.c_int, [P, i32, ctypes.POINTER(ContentParams), i32]),
         "h261g_stream_frames_done": (ctypes.c_int, [P, i32]),
         "h261g_stream_input_pending": (ctypes.c_int, [P, i32]),

@@ -115,6 +115,33 @@ __global__ void remap_kernel(StreamDesc *desc, int32_t *mb_stream, const int32_t
     if (threadIdx.x == 0) desc[s].mb_base = base;
 }
 
+/* host-fed input of up to kPutIds streams, packed in one staging buffer:
+ * frame i (I420, planes back to back) at stage + off[i] goes to its
+ * stream's input buffer (desc.cur[0], planes back to back as well), and the
+ * descriptor becomes active with the device quantiser (what
+ * h261g_stream_put_frame writes with a copy).  Offsets are multiples of 16
+ * and frame sizes are too (CIF and QCIF), so the copy moves 16-byte words.
+ * Blocks (x, i): frame i, a grid-stride share of its words. */
+constexpr int kPutIds = 256;
+struct PutFrames {
+    const uint8_t *stage;
+    int32_t n;
+    int32_t id[kPutIds];
+    int64_t off[kPutIds];
+};
+static_assert(sizeof(PutFrames) <= 4096 - 64, "kernel argument size");
+__global__ void put_frames_kernel(StreamDesc *desc, PutFrames a)
+{
+    const int i = blockIdx.y;
+    if (i >= a.n) return;
+    StreamDesc &d = desc[a.id[i]];
+    const int words = d.width * d.height * 3 / 32; /* 16-byte words of an I420 frame */
+    const uint4 *src = (const uint4 *)(a.stage + a.off[i]);
+    uint4 *dst = (uint4 *)d.cur[0];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < words; k += gridDim.x * blockDim.x) dst[k] = src[k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.flags = kStreamActive | kStreamDeviceQuant;
+}
+
 /* a closed stream: inactive descriptor, MB range unmapped */
 __global__ void stream_close_kernel(StreamDesc *desc, int32_t *mb_stream, int id, int base, int n_mb)
 {
@@ -170,6 +197,17 @@ struct h261g_engine {
     hipStream_t chain_stream = nullptr;
     hipEvent_t chain_fork = nullptr, chain_join = nullptr;
     hipEvent_t chain_me_a = nullptr, chain_me_b[2] = {nullptr, nullptr};
+    /* h261g_engine_put_frames: two device staging buffers of the packed host
+     * frames, used in turn; the copies run on put_stream (default priority)
+     * beside the engine stream's steps.  put_copied[b]: buffer b's copy done
+     * (the engine stream waits for it before placing the frames);
+     * put_placed[b]: the kernel that read buffer b done (the next copy into
+     * b waits for it) */
+    uint8_t *put_stage[2] = {nullptr, nullptr};
+    size_t put_stage_cap[2] = {0, 0};
+    int put_calls = 0;
+    hipStream_t put_stream = nullptr;
+    hipEvent_t put_copied[2] = {nullptr, nullptr}, put_placed[2] = {nullptr, nullptr};
     int chains = 0; /* 0 by size, 1 one chain, 2 two chains */
     int ladder = 0; /* two chains: 0 by size, 1 strict, 2 one step of slack */
     int async_input = 0;
@@ -825,6 +863,13 @@ int h261g_engine_destroy(h261g_engine_t *e)
     hipHostFree(e->fetch_meta);
     hipHostFree(e->fetch_data);
     hipFree(e->fetch_dev);
+    if (e->put_stream) (void)hipStreamSynchronize(e->put_stream);
+    for (int b = 0; b < 2; b++) {
+        hipFree(e->put_stage[b]);
+        if (e->put_copied[b]) (void)hipEventDestroy(e->put_copied[b]);
+        if (e->put_placed[b]) (void)hipEventDestroy(e->put_placed[b]);
+    }
+    if (e->put_stream) (void)hipStreamDestroy(e->put_stream);
     if (e->fetch_ev) (void)hipEventDestroy(e->fetch_ev);
     for (int r = 0; r < h261g_engine::kEvRing; r++)
         for (int i = 0; i < 8; i++)
@@ -998,6 +1043,74 @@ int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_t *y, cons
                 ok(hipStreamSynchronize(e->stream), "sync");
     s->pending = true;
     return good ? H261G_OK : H261G_ERR_DEVICE;
+}
+
+int h261g_engine_put_frames(h261g_engine_t *e, int32_t n, const int32_t *ids, const uint8_t *frames,
+                            int64_t frames_bytes, const int64_t *offsets)
+{
+    if (!e || n < 0 || (n > 0 && (!ids || !frames || !offsets))) return H261G_ERR_ARG;
+    /* everything is checked before anything is queued: open host-fed
+     * streams, each at most once, 16-byte aligned offsets, frames inside
+     * the buffer */
+    std::vector<uint8_t> seen(e->slots.size(), 0);
+    int64_t lo = INT64_MAX, hi = 0;
+    for (int i = 0; i < n; i++) {
+        Slot *s = get_slot(e, ids[i]);
+        if (!s || s->seq || seen[ids[i]] || offsets[i] < 0 || (offsets[i] & 15)) return H261G_ERR_ARG;
+        seen[ids[i]] = 1;
+        lo = offsets[i] < lo ? offsets[i] : lo;
+        const int64_t end = offsets[i] + (int64_t)s->width * s->height * 3 / 2;
+        if (end > frames_bytes) return H261G_ERR_ARG;
+        hi = end > hi ? end : hi;
+    }
+    if (n == 0) return H261G_OK;
+    if (flush_writer(e)) return H261G_ERR_DEVICE;
+    if (!e->put_stream) {
+        if (!ok(hipStreamCreateWithFlags(&e->put_stream, hipStreamNonBlocking), "stream")) return H261G_ERR_DEVICE;
+        for (int b = 0; b < 2; b++)
+            if (!ok(hipEventCreateWithFlags(&e->put_copied[b], hipEventDisableTiming), "event") ||
+                !ok(hipEventCreateWithFlags(&e->put_placed[b], hipEventDisableTiming), "event"))
+                return H261G_ERR_DEVICE;
+    }
+    const int b = e->put_calls & 1;
+    const size_t span = (size_t)(hi - lo);
+    if (span > e->put_stage_cap[b]) {
+        /* the previous copy into buffer b and the kernel that read it may
+         * still run */
+        if (!ok(hipStreamSynchronize(e->put_stream), "sync") || !ok(hipStreamSynchronize(e->stream), "sync"))
+            return H261G_ERR_DEVICE;
+        if (e->put_stage[b]) (void)hipFree(e->put_stage[b]);
+        e->put_stage[b] = nullptr;
+        e->put_stage_cap[b] = 0;
+        const size_t cap = span + span / 4;
+        if (!ok(hipMalloc((void **)&e->put_stage[b], cap), "hipMalloc")) return H261G_ERR_MEMORY;
+        e->put_stage_cap[b] = cap;
+    }
+    /* one copy of the whole span (a per-plane copy costs microseconds of
+     * runtime overhead each, far more than its bytes take on PCIe), on the
+     * copy stream once the frames last staged in buffer b are placed, so it
+     * runs beside the steps queued so far */
+    if (e->put_calls >= 2 && !ok(hipStreamWaitEvent(e->put_stream, e->put_placed[b], 0), "wait"))
+        return H261G_ERR_DEVICE;
+    if (!ok(hipMemcpyAsync(e->put_stage[b], frames + lo, span, hipMemcpyHostToDevice, e->put_stream), "hipMemcpy") ||
+        !ok(hipEventRecord(e->put_copied[b], e->put_stream), "event") ||
+        !ok(hipStreamWaitEvent(e->stream, e->put_copied[b], 0), "wait"))
+        return H261G_ERR_DEVICE;
+    for (int i0 = 0; i0 < n; i0 += kPutIds) {
+        PutFrames a{};
+        a.stage = e->put_stage[b];
+        a.n = min(kPutIds, n - i0);
+        for (int k = 0; k < a.n; k++) {
+            a.id[k] = ids[i0 + k];
+            a.off[k] = offsets[i0 + k] - lo;
+        }
+        hipLaunchKernelGGL(put_frames_kernel, dim3(8, a.n), dim3(256), 0, e->stream, e->desc, a);
+        if (!ok(hipGetLastError(), "put_frames")) return H261G_ERR_DEVICE;
+    }
+    if (!ok(hipEventRecord(e->put_placed[b], e->stream), "event")) return H261G_ERR_DEVICE;
+    e->put_calls++;
+    for (int i = 0; i < n; i++) e->slots[ids[i]].pending = true;
+    return H261G_OK;
 }
 
 int h261g_stream_input_pending(h261g_engine_t *e, int32_t id)

@@ -89,6 +89,37 @@ class Engine:
                                               ptr(np.ascontiguousarray(cb)), ptr(np.ascontiguousarray(cr))),
               "put_frame")
 
+    def put_frames(self, sids, buf: np.ndarray, offsets) -> None:
+        """Input of the next step for several host-fed streams at once
+        (h261g_engine_put_frames): stream sids[i]'s I420 frame (Y, Cb, Cr
+        back to back) starts at byte offsets[i] of the uint8 buffer buf.  The
+        copy is queued, not waited for: a buffer in pinned memory must stay
+        unchanged until the next step has finished."""
+        ids = np.ascontiguousarray(np.asarray(sids, dtype=np.int32))
+        offs = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        if buf.dtype != np.uint8 or buf.ndim != 1 or not buf.flags["C_CONTIGUOUS"]:
+            raise ValueError("put_frames: a contiguous one-dimensional uint8 buffer")
+        if len(ids) != len(offs):
+            raise ValueError("put_frames: one offset per stream id")
+        check(self.lib.h261g_engine_put_frames(self.h, len(ids), ptr(ids), ptr(buf), buf.size, ptr(offs)),
+              "put_frames")
+
+    @staticmethod
+    def pack_frames(frames, align: int = 16):
+        """Packs (y, cb, cr) frames into one uint8 buffer (each frame's planes
+        back to back, frames at multiples of `align`); returns (buf, offsets)."""
+        offs, pos = [], 0
+        for y, cb, cr in frames:
+            offs.append(pos)
+            pos += -(-(y.size + cb.size + cr.size) // align) * align
+        buf = np.zeros(pos, np.uint8)
+        for (y, cb, cr), o in zip(frames, offs):
+            n0, n1 = y.size, y.size + cb.size
+            buf[o:o + n0] = y.reshape(-1)
+            buf[o + n0:o + n1] = cb.reshape(-1)
+            buf[o + n1:o + n1 + cr.size] = cr.reshape(-1)
+        return buf, offs
+
     def generate_input(self, sid: int, seed: int, n_frames: int, sigma: int = 2, motion: int = 7,
                        local: bool = False, scene_cut: int = 0) -> None:
         cp = nat.ContentParams(seed=seed, sigma=sigma, motion=motion, local=int(local), scene_cut=scene_cut)

@@ -191,6 +191,22 @@ H261G_API int h261g_stream_close(h261g_engine_t *e, int32_t id);
 /* input of the next step: copy host planes (Y, Cb, Cr with stride = width
  * resp. width/2) */
 H261G_API int h261g_stream_put_frame(h261g_engine_t *e, int32_t id, const uint8_t *y, const uint8_t *cb, const uint8_t *cr);
+/* input of the next step for n host-fed streams at once, from one host
+ * buffer of frames_bytes bytes: stream ids[i]'s frame is the I420 frame at
+ * frames + offsets[i] (Y, Cb, Cr planes back to back, strides width and
+ * width/2; offsets multiples of 16, every frame inside the buffer).  One copy of the span from the lowest offset to the end
+ * of the highest frame crosses PCIe into a device staging buffer of the
+ * engine (grown on demand, kept until h261g_engine_destroy), and one kernel
+ * moves each frame into its stream's input buffer and activates the stream;
+ * nothing waits.  From pageable memory the runtime stages the span before
+ * the call returns; from pinned memory (hipHostMalloc) it is read
+ * asynchronously and must stay unchanged until the next step has finished
+ * (h261g_engine_sync, a read, or a completed fetch).  Every id must name an
+ * open stream without device-resident input, at most once; otherwise
+ * H261G_ERR_ARG and nothing is queued.  Frames may share bytes (they are
+ * only read). */
+H261G_API int h261g_engine_put_frames(h261g_engine_t *e, int32_t n, const int32_t *ids, const uint8_t *frames,
+                                      int64_t frames_bytes, const int64_t *offsets);
 /* device-resident input: generates frames 0..n_frames-1 of synthetic
  * content into HBM; the stream then codes one per step until the sequence
  * ends (the stream stays open, inactive, afterwards) */

@@ -310,6 +310,111 @@ def test_engine_graph_run_matches_stepwise():
     b.destroy()
 
 
+def test_put_frames_matches_put_frame(oracle):
+    """Host-fed input for many streams in one call (h261g_engine_put_frames:
+    the batch packed in one host buffer, one copy and one kernel, nothing
+    waits) codes exactly what one h261g_stream_put_frame per stream codes,
+    from pageable and from pinned host memory, with frames in any order and
+    some streams skipping a step; one stream also against the oracle.  An
+    invalid call queues nothing."""
+    import torch
+    specs = [(CIF, 501, 0, 256000), (QCIF, 502, 0, 96000), (QCIF, 503, 8, 0), (CIF, 504, 8, 0)]
+    n = 6
+    extra = 3
+    frames = [seq(seed, size, n + extra, sigma=2, motion=9) for size, seed, _, _ in specs]
+    engines = []
+    for _ in range(3):
+        eng = Engine(max_streams=8)
+        sids = [eng.open(*size, fixed_quant=fixed, bitrate=rate or 512000, vbv_size=rate or 512000)
+                for size, _, fixed, rate in specs]
+        engines.append((eng, sids))
+    (a, ia), (b, ib), (c, ic) = engines
+    announce("put_frames: per-stream, packed pageable, packed pinned")
+
+    size0, _, fixed0, rate0 = specs[0]
+    os_ = oracle.stream(*size0, fixed_quant=fixed0, bitrate=rate0, vbv=rate0)
+    # stream 1 skips step 2, stream 2 skips step 3 (no input: not coded)
+    skip = {(1, 2), (2, 3)}
+    done = [0] * len(specs)
+    for t in range(n):
+        fed = [k for k in range(len(specs)) if (k, t) not in skip]
+        for k in fed:
+            a.put_frame(ia[k], *frames[k][done[k]])
+        # packed in reverse order, so offsets do not follow the ids
+        order = fed[::-1]
+        buf, offs = Engine.pack_frames([frames[k][done[k]] for k in order])
+        b.put_frames([ib[k] for k in order], buf, offs)
+        pin = torch.empty(buf.size, dtype=torch.uint8, pin_memory=True)
+        pbuf = pin.numpy()
+        pbuf[...] = buf
+        c.put_frames([ic[k] for k in order], pbuf, offs)
+        for eng, _ in engines:
+            eng.step()
+            eng.sync()
+        del pin, pbuf  # the pinned buffer may go once the step has finished
+        for k in fed:
+            want = a.bitstream(ia[k])
+            assert b.bitstream(ib[k]) == want, f"packed pageable, step {t} stream {k}"
+            assert c.bitstream(ic[k]) == want, f"packed pinned, step {t} stream {k}"
+            assert a.info(ia[k]).quant == b.info(ib[k]).quant == c.info(ic[k]).quant
+            ra, rb, rc = a.recon(ia[k]), b.recon(ib[k]), c.recon(ic[k])
+            for p in range(3):
+                assert np.array_equal(ra[p], rb[p]) and np.array_equal(ra[p], rc[p]), f"step {t} stream {k} recon"
+            done[k] += 1
+        ob = os_.encode(*frames[0][t])[0]
+        assert c.bitstream(ic[0]) == ob, f"packed pinned vs oracle, frame {t}"
+    for k, (eng, sids) in enumerate(engines):
+        assert [eng.frames_done(s) for s in sids] == done, f"engine {k}"
+
+    # steps queued back to back: each call's copy runs beside the steps
+    # queued before it, into the staging buffer the call before last used;
+    # nothing is synchronised until the end (every pinned buffer kept)
+    keep = []
+    for t in range(extra):
+        for k in range(len(specs)):
+            a.put_frame(ia[k], *frames[k][done[k] + t])
+        a.step()
+        a.sync()
+        buf, offs = Engine.pack_frames([frames[k][done[k] + t] for k in range(len(specs))])
+        pin = torch.empty(buf.size, dtype=torch.uint8, pin_memory=True)
+        pin.numpy()[...] = buf
+        keep.append(pin)
+        c.put_frames(ic, pin.numpy(), offs)
+        c.step()
+    c.sync()
+    del keep
+    done = [d + extra for d in done]
+    for k in range(len(specs)):
+        assert c.bitstream(ic[k]) == a.bitstream(ia[k]), f"queued steps: stream {k}"
+        ra, rc = a.recon(ia[k]), c.recon(ic[k])
+        for p in range(3):
+            assert np.array_equal(ra[p], rc[p]), f"queued steps: stream {k} recon"
+    assert [c.frames_done(s) for s in ic] == done
+
+    # invalid calls queue nothing: a repeated id, a stream with device input,
+    # an id that is not open, an offset that is not a multiple of 16, a
+    # frame past the buffer's end
+    buf, offs = Engine.pack_frames([frames[0][0], frames[0][1]])  # two CIF frames
+    with pytest.raises(RuntimeError):
+        b.put_frames([ib[0], ib[0]], buf, offs)
+    gen = b.open(*QCIF)
+    b.generate_input(gen, 7, 2)
+    with pytest.raises(RuntimeError):
+        b.put_frames([ib[1], gen], buf, offs)
+    with pytest.raises(RuntimeError):
+        b.put_frames([ib[1], 7], buf, offs)
+    with pytest.raises(RuntimeError):
+        b.put_frames([ib[1]], buf, [8])
+    with pytest.raises(RuntimeError):
+        b.put_frames([ib[0], ib[3]], buf, [offs[1], offs[1] + 16])
+    b.step()
+    b.sync()
+    assert [b.frames_done(s) for s in ib] == [d - extra for d in done], "a rejected call activated a stream"
+    assert b.frames_done(gen) == 1
+    for eng, _ in engines:
+        eng.destroy()
+
+
 def test_engine_two_chains_match_one_chain(oracle):
     """Two chains (the MB space split at a stream boundary, each half stepped
     on its own HIP stream), tied by the strict ladder or by the ladder with
