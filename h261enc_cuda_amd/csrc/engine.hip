@@ -1066,11 +1066,22 @@ int h261g_engine_put_frames(h261g_engine_t *e, int32_t n, const int32_t *ids, co
     if (n == 0) return H261G_OK;
     if (flush_writer(e)) return H261G_ERR_DEVICE;
     if (!e->put_stream) {
-        if (!ok(hipStreamCreateWithFlags(&e->put_stream, hipStreamNonBlocking), "stream")) return H261G_ERR_DEVICE;
-        for (int b = 0; b < 2; b++)
-            if (!ok(hipEventCreateWithFlags(&e->put_copied[b], hipEventDisableTiming), "event") ||
-                !ok(hipEventCreateWithFlags(&e->put_placed[b], hipEventDisableTiming), "event"))
-                return H261G_ERR_DEVICE;
+        /* the copy stream and its events are kept only when all exist */
+        hipStream_t st = nullptr;
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        bool good = ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+        for (int k = 0; good && k < 4; k++) good = ok(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming), "event");
+        if (!good) {
+            for (hipEvent_t x : ev)
+                if (x) (void)hipEventDestroy(x);
+            if (st) (void)hipStreamDestroy(st);
+            return H261G_ERR_DEVICE;
+        }
+        e->put_stream = st;
+        e->put_copied[0] = ev[0];
+        e->put_copied[1] = ev[1];
+        e->put_placed[0] = ev[2];
+        e->put_placed[1] = ev[3];
     }
     const int b = e->put_calls & 1;
     const size_t span = (size_t)(hi - lo);
