@@ -76,3 +76,20 @@ def test_poisson_request_mix():
     assert 100 < np.median(lens) < 140 and lens.max() <= 1800
     rates = np.array([r.bitrate for r in reqs])
     assert rates.min() >= 64000 and rates.max() <= 1024000
+
+
+def test_pack_frames_layout():
+    """Engine.pack_frames (the host buffer of h261g_engine_put_frames): each
+    frame's Y, Cb, Cr back to back, frames at multiples of 16, nothing
+    overlapping."""
+    from h261enc_cuda_amd import Engine
+    frames = [content_frame(s, w, h, 0) for s, (w, h) in ((1, (352, 288)), (2, (176, 144)), (3, (176, 144)))]
+    buf, offs = Engine.pack_frames(frames)
+    assert buf.dtype == np.uint8 and buf.ndim == 1
+    assert all(o % 16 == 0 for o in offs)
+    ends = [o + sum(p.size for p in f) for o, f in zip(offs, frames)]
+    assert all(e <= o2 for e, o2 in zip(ends, offs[1:])) and ends[-1] <= buf.size
+    for (y, cb, cr), o in zip(frames, offs):
+        assert np.array_equal(buf[o:o + y.size].reshape(y.shape), y)
+        assert np.array_equal(buf[o + y.size:o + y.size + cb.size].reshape(cb.shape), cb)
+        assert np.array_equal(buf[o + y.size + cb.size:o + y.size + cb.size + cr.size].reshape(cr.shape), cr)
