@@ -9,14 +9,16 @@
  *   lanes 0-47 = (block b, row r):   MC / loop filter, residual, DCT rows
  *   LDS transpose
  *   lanes 0-47 = (block b, col c):   DCT columns, denoise, quantiser, CBP
- *   lanes 0-47 = (block b, row r):   an uncoded block's reconstruction is
- *                                    its prediction (all-zero levels give a
- *                                    zero IDCT, mbgpu:966-976); a coded
- *                                    block keeps its prediction in LDS
+ *   lanes 0-47 = (block b, row r):   every block's prediction goes to the
+ *                                    reconstruction plane: an uncoded
+ *                                    block's reconstruction is its
+ *                                    prediction (all-zero levels give a
+ *                                    zero IDCT, mbgpu:966-976)
  * and then, for the coded blocks only, packed 8 to a pass:
  *   lane (slot, col c):              dequantiser, IDCT columns
  *   LDS transpose
- *   lane (slot, row r):              IDCT rows, reconstruction
+ *   lane (slot, row r):              IDCT rows, the prediction row read
+ *                                    back, reconstruction
  *
  * Arithmetic follows the reference op for op (compiled with
  * -ffp-contract=off and '#pragma clang fp contract(off)'): products with an
@@ -191,12 +193,14 @@ __device__ __forceinline__ int wave_sum_i32(int x)
            __builtin_amdgcn_readlane(x, 48);
 }
 
-/* bits of one TCOEFF event: the code and its sign bit, or an escape; tlen
- * is the wave's LDS copy of the code lengths plus one (kEscapeBits where the
- * table has no code) */
-__device__ __forceinline__ int tcoeff_bits(const uint8_t *tlen, int run, int level)
+/* bits of one TCOEFF event: the code and its sign bit, or an escape; the
+ * code lengths plus one (kEscapeBits where the table has no code) are read
+ * from constant memory (only MBs with coded blocks count bits, and an LDS
+ * copy would cost the coder a wave per SIMD) */
+__device__ __forceinline__ int tcoeff_bits(int run, int level)
 {
     const int a = level < 0 ? -level : level;
+    const uint8_t *tlen = (const uint8_t *)c_tlen.w;
     return (run < kTcoeffRuns && a < kTcoeffLevels) ? (int)tlen[run * kTcoeffLevels + a] : kEscapeBits;
 }
 
@@ -316,21 +320,23 @@ __device__ __forceinline__ void flush_stats(const MbLaunch &p, int s, int32_t (*
  * inter blocks are coded, so mostly one pass or none).  Last, per MB of the
  * group, the CBP, bit count and outputs. */
 template <int G>
-__global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
+/* 6 waves per SIMD: 80 VGPRs and 5.9 KB of LDS per wave (at 7.9 KB, with the
+ * prediction rows and the TCOEFF lengths in LDS, the LDS held it at 5).  The
+ * two-chain step gains more than the coder alone (0.331 -> 0.318 ms per full
+ * step, 133 -> 131 us per launch): the other half's search waves fit beside
+ * the coder's */
+__global__ __launch_bounds__(64, 6) void mb_kernel(MbLaunch p)
 {
     constexpr int kBlocks = 6 * G;
     constexpr int kPasses = (kBlocks + 7) / 8;
     __shared__ float tf[8 * kBlkPitch];
     __shared__ __align__(16) int8_t lv[kBlocks * 64]; /* levels; an intra DC (1..255) as uint8 */
-    __shared__ uint2 pred_lds[kBlocks * 8];          /* coded blocks: prediction rows */
     __shared__ int32_t stat[2][2][64]; /* [stream slot][intra, inter][coefficient] */
     __shared__ int32_t scount[2][2];
-    __shared__ __align__(16) uint32_t tlen32[kTlenWords];
     __shared__ MbInfo info[G];
     /* the previous frame's denoise offsets staged as UInt16 (mbgpu:525-528):
      * [stream slot][intra, inter][coefficient] */
     __shared__ uint16_t dn_off[2][2][64];
-    const uint8_t *tlen = (const uint8_t *)tlen32;
 
     const int lane = threadIdx.x;
     const int k8 = lane >> 3; /* block of the pass */
@@ -338,7 +344,6 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 
     stat[0][0][lane] = stat[0][1][lane] = stat[1][0][lane] = stat[1][1][lane] = 0;
     if (lane < 4) scount[lane >> 1][lane & 1] = 0;
-    if (lane < kTlenWords / 2) ((uint2 *)tlen32)[lane] = ((const uint2 *)c_tlen.w)[lane];
     /* per-lane constants: zig-zag position of lane, column scale factors */
     const int zz = c_zigzag[lane];
     const float post_c = (float)((double)c_fwd_post[rc] * 0.125);
@@ -540,14 +545,15 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             for (int k = 0; k < 8; k++)
                 if ((coded_mask >> (8 * k)) & 1) coded_bits |= 1u << (pass * 8 + k);
 
-            /* lane (block, row rc) holds row rc of the block's prediction */
+            /* lane (block, row rc) holds row rc of the block's prediction: the
+             * reconstruction of an uncoded block, and for a coded block the
+             * prediction its residual is added to in the coded-block passes
+             * below, which read it back (the barriers between order the
+             * wave's global stores and loads).  Kept in LDS instead, the
+             * coder would hold a wave per SIMD less */
             if (L.act) {
                 uint8_t *dst = info[L.mi].rec[L.plane] + (size_t)L.py * L.stride + L.px;
-                if (!coded) {
-                    store_global2(dst + (size_t)rc * L.stride, predw[0], predw[1]);
-                } else {
-                    pred_lds[bi * 8 + rc] = make_uint2(predw[0], predw[1]);
-                }
+                store_global2(dst + (size_t)rc * L.stride, predw[0], predw[1]);
             }
             __syncthreads(); /* tf is reused by the next pass */
         }
@@ -570,6 +576,11 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
             const LaneBlock C = lane_block(info, act ? bi : 0, kBlocks, s0);
             const bool intra = C.intra;
             const int qmul = C.quant * 2, qadd = (C.quant - 1) | 1;
+            /* the prediction row this lane adds to in the row pass, loaded now
+             * so the load overlaps the column pass */
+            uint32_t predw[2] = {0, 0};
+            uint8_t *rdst = info[C.mi].rec[C.plane] + (size_t)(C.py + rc) * C.stride + C.px;
+            if (act) load_global2(rdst, predw[0], predw[1]);
             if (act) {
                 const int c = rc;
                 /* dequantiser (mbgpu:763-814) */
@@ -607,8 +618,6 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
 #pragma unroll
                 for (int k = 0; k < 8; k++) v[k] = tf[k8 * kBlkPitch + rc * 8 + k];
                 idct_core(v, o8);
-                const uint2 pw = pred_lds[bi * 8 + rc];
-                const uint32_t predw[2] = {pw.x, pw.y};
                 uint32_t w[2] = {0, 0};
 #pragma unroll
                 for (int x = 0; x < 8; x++) {
@@ -617,8 +626,7 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                     const int val = clampi(pv + rres, 0, 255);
                     w[x >> 2] |= (uint32_t)val << (8 * (x & 3));
                 }
-                uint8_t *dst = info[C.mi].rec[C.plane] + (size_t)(C.py + rc) * C.stride + C.px;
-                store_global2(dst, w[0], w[1]);
+                store_global2(rdst, w[0], w[1]);
             }
             __syncthreads(); /* tf and slots are reused by the next pass */
         }
@@ -667,9 +675,9 @@ __global__ __launch_bounds__(64, 5) void mb_kernel(MbLaunch p)
                             else if (l == 1 || l == -1)
                                 bits += 2; /* first coefficient '1s' */
                             else
-                                bits += tcoeff_bits(tlen, 0, l);
+                                bits += tcoeff_bits(0, l);
                         } else {
-                            bits += tcoeff_bits(tlen, run, l);
+                            bits += tcoeff_bits(run, l);
                         }
                     }
                     if (lane == 0) bits += 2; /* EOB */
