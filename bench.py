@@ -58,11 +58,12 @@ MB_BYTES = {396: 3 * 152064 + 396 * (24 + 384 + 8), 99: 3 * 38016 + 99 * (24 + 3
 # (profiles/batched_512_streams_hbm_traffic.md).  Motion estimation depends on
 # its run and tile (R, T): chosen per launch size (0, 0; runs of 2 MBs in one
 # tile at these launches) 80.3 MB read + 7.7 MB written, runs of 4 in tiles of
-# 2: 74.2 + 9.3 MB.  The coder 107.0 + 126.7 MB, the bitstream writer (not
-# remapped) 20.1 + 6.5 MB.  Other launch shapes: not measured.
+# 2: 74.2 + 9.3 MB.  The coder 131.8 + 148.6 MB (at 6 waves per SIMD, with
+# more MBs in flight per XCD; 107.0 + 126.7 MB at 5), the bitstream writer
+# (not remapped) 20.1 + 6.5 MB.  Other launch shapes: not measured.
 TRAFFIC_PER_MB = {("me_kernel", True, (0, 0)): (80.30e6 + 7.66e6) / 126720,
                   ("me_kernel", True, (4, 2)): (74.22e6 + 9.28e6) / 126720,
-                  ("mb_kernel", True, None): (106.96e6 + 126.66e6) / 126720,
+                  ("mb_kernel", True, None): (131.84e6 + 148.56e6) / 126720,
                   ("bitstream_kernel", False, None): (20.10e6 + 6.47e6) / 126720}
 REMAP_BIT = {"me_kernel": 1, "mb_kernel": 2, "bitstream_kernel": 4}
 

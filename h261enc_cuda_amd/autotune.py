@@ -20,9 +20,10 @@ what it computes:
 The tuner times candidates on a sample of the target workload -- the same
 mix of stream sizes, all streams active, inter frames -- with HIP events
 around every launch of the step, and keeps the fastest: first over
-`mbs_per_wave` at the default remap, then over the pairs (`me_mbs_per_wave`,
-`me_tile`), then over the remap at the best of both (coordinate descent; the
-knobs act on mostly different kernels).  The motion-estimation shape chosen
+`mbs_per_wave` at the default remap (by the coder's own time, the one kernel
+it changes), then over the pairs (`me_mbs_per_wave`, `me_tile`), then over
+the remap at the best of both (coordinate descent; the knobs act on mostly
+different kernels).  The motion-estimation shape chosen
 per launch (0, 0) is kept unless a fixed shape beats it by more than twice
 `tie`: the sample is at full load, where the shapes differ by less than their
 noise (about 1.5%), while over a job whose streams finish the per-launch
@@ -64,7 +65,7 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
     key = (tuple(sizes), tuple(mpw_c), tuple(remap_c), tuple(me_shapes), steps)
     if key in _CACHE:
         return _CACHE[key]
-    frames = 2 + (steps + 1) * (len(mpw_c) + len(remap_c) + 2 * len(me_shapes))
+    frames = 2 + (steps + 1) * (2 * len(mpw_c) + len(remap_c) + 2 * len(me_shapes))
     eng = Engine(max_streams=len(sizes))
     # kernels alone (the writer at the end of its step, not beside the next
     # step's motion estimation), so the event spans add up to the step
@@ -99,10 +100,17 @@ def tune(sizes: Sequence[Tuple[int, int]], mbs_per_wave: Iterable[int] = (0, 1, 
             return ms
 
         default_remap = 3
-        # within the tie band a fixed count wins over 0 ("by batch size"):
-        # the sample is at full load, where 0 picks the same shape, but as
-        # streams finish 0 changes shape and a job ran about 7% slower
-        mpw_t = {m: trial(m, default_remap) for m in mpw_c}
+        # the count changes only the coder, so it is judged by the coder's
+        # own time, each count twice in palindromic order: the step time
+        # also carries motion estimation's noise, and one slow search
+        # (+7%) once made a job pick 0.  Within the tie band a fixed count
+        # wins over 0 ("by batch size"): the sample is at full load, where
+        # 0 picks the same shape, but as streams finish 0 changes shape and
+        # a job ran about 5-7% slower
+        mpw_t = {m: 0.0 for m in mpw_c}
+        for m in mpw_c + mpw_c[::-1]:
+            trial(m, default_remap)
+            mpw_t[m] += trials[-1]["mb_ms"] / 2
         mpw_fast = min(mpw_t.values())
         mpw_close = [m for m in mpw_c if mpw_t[m] <= mpw_fast * (1 + tie)]
         best_mpw = min((m for m in mpw_close if m), key=lambda m: mpw_t[m],

@@ -35,7 +35,7 @@ def test_autotune_picks_a_measured_candidate_and_keeps_results():
     r = autotune.tune(sizes, mbs_per_wave=(1, 4), xcd_remap=(0, 2), me_mbs_per_wave=(1, 2), me_tile=(1, 2), steps=2)
     assert r["mbs_per_wave"] in (1, 4) and r["xcd_remap"] in (0, 2)
     assert (r["me_mbs_per_wave"], r["me_tile"]) in ((0, 0), (1, 1), (2, 1), (2, 2))
-    assert len(r["trials"]) == 2 + 2 * 4 + 2 and all(t["step_ms"] > 0 for t in r["trials"])
+    assert len(r["trials"]) == 2 * 2 + 2 * 4 + 2 and all(t["step_ms"] > 0 for t in r["trials"])
     key = ("mbs_per_wave", "xcd_remap", "me_mbs_per_wave", "me_tile")
     pick = tuple(r[k] for k in key)
     assert r["step_ms"] == min(t["step_ms"] for t in r["trials"] if tuple(t[k] for k in key) == pick)
