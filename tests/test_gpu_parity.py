@@ -479,6 +479,48 @@ def test_engine_two_chains_match_one_chain(oracle):
         eng.destroy()
 
 
+@pytest.mark.skip(reason="two-chain steps with each chain's writer on a side stream beside its next search "
+                  "are switched off: their first graph capture ended the process with a host segfault in "
+                  "h261g_engine_run (the second chain's writer stream forked from the second chain's stream); "
+                  "cause not found")
+def test_engine_two_chains_overlapped_writers_match_one_chain():
+    """Two chains whose bitstream writers run beside each chain's next search
+    (OPT_OVERLAP_WRITER 1) code every stream as one chain does, stepwise and
+    under graph replay."""
+    from h261enc_cuda_amd._native import OPT_CHAIN_LADDER, OPT_CHAINS, OPT_FUSED_TAILS, OPT_OVERLAP_WRITER
+    specs = [(CIF, 301, 0, 256000), (QCIF, 302, 0, 64000), (QCIF, 303, 6, 0), (CIF, 304, 0, 384000),
+             (QCIF, 305, 0, 128000)]
+    engines = []
+    for chains, ladder, overlap in ((1, 0, 2), (2, 1, 1), (2, 2, 1)):
+        eng = Engine(max_streams=8)
+        eng.config(OPT_CHAINS, chains)
+        eng.config(OPT_CHAIN_LADDER, ladder)
+        eng.config(OPT_OVERLAP_WRITER, overlap)
+        if chains == 2:
+            eng.config(OPT_FUSED_TAILS, 2)
+        sids = []
+        for size, seed, fixed, rate in specs:
+            sid = eng.open(*size, fixed_quant=fixed, bitrate=rate or 512000, vbv_size=rate or 512000)
+            eng.generate_input(sid, seed, 20, sigma=2, motion=15)
+            sids.append(sid)
+        engines.append((eng, sids))
+    announce("engine two chains, overlapped writers, vs one chain")
+    for what, steps in (("stepwise", 4), ("graph replay", 5)):
+        for eng, _ in engines:
+            if what == "stepwise":
+                for _ in range(steps):
+                    eng.step()
+            else:
+                eng.run(steps)
+            eng.sync()
+        a, ia = engines[0]
+        for j, (b, ib) in enumerate(engines[1:], 1):
+            for k, (sa, sb) in enumerate(zip(ia, ib)):
+                assert a.bitstream(sa) == b.bitstream(sb), f"{what}: engine {j} stream {k} bitstream"
+    for eng, _ in engines:
+        eng.destroy()
+
+
 @pytest.mark.parametrize("fused", [1, 2], ids=["rc-begin-in-me", "rc-begin-launch"])
 def test_engine_me_runs_agree(oracle, fused):
     """Motion estimation with 1, 2, 4 or 8 consecutive MBs per wave (runs
