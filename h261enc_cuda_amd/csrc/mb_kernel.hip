@@ -23,7 +23,8 @@
  * Arithmetic follows the reference op for op (compiled with
  * -ffp-contract=off and '#pragma clang fp contract(off)'): products with an
  * unsuffixed literal are double multiplies rounded to float; the forward
- * DCT rounds with floorf((float)(x + 0.49999)), the IDCT with
+ * DCT rounds with floorf((float)(x + 0.49999)) (computed as the FP32 sum
+ * x + 0.49999f, the same for every float x in range), the IDCT with
  * floor((double)x + 0.49999) (src/mbgpu_kernels.cuh:337-442, 827-931).
  * A wave codes groups of consecutive MBs; per group, each MB's stream,
  * position, quantiser, mode and plane pointers, and the denoise offset
@@ -496,9 +497,14 @@ __global__ __launch_bounds__(64, 6) void mb_kernel(MbLaunch p)
                 float o8[8];
                 fdct_core(v[0] + v[7], v[1] + v[6], v[2] + v[5], v[3] + v[4], v[3] - v[4], v[2] - v[5],
                           v[1] - v[6], v[0] - v[7], o8);
+                /* the reference's floorf(x + 0.49999): a double sum rounded
+                 * to float, then floored.  For every float x with |x| <=
+                 * 1e7 (coefficients stay below 2^13) that equals floorf of
+                 * the FP32 sum x + 0.49999f (checked over every float bit
+                 * pattern), so no conversions or FP64 add are needed */
                 float cf[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) cf[k] = floorf((float)((double)(cs * kFwdPost[k] * o8[k]) + 0.49999));
+                for (int k = 0; k < 8; k++) cf[k] = floorf(cs * kFwdPost[k] * o8[k] + 0.49999f);
 
                 /* denoise with the previous frame's offsets, staged as UInt16
                  * (mbgpu:525-528, 586-604); the statistics stay integer LDS
