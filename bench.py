@@ -325,6 +325,9 @@ def main():
                     help="also run a 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--chain-ladder", type=int, default=0, choices=(0, 1, 2),
                     help="two-chain ladder (H261G_OPT_CHAIN_LADDER): 0 by size, 1 strict, 2 one step of slack")
+    ap.add_argument("--fused-tails", type=int, default=3, choices=(0, 1, 2, 3),
+                    help="rate-control begin (H261G_OPT_FUSED_TAILS): 1 in the tail of motion estimation, "
+                         "2 as its own launch, 3 by size (default)")
     ap.add_argument("--compact-every", type=int, default=30,
                     help="steps between closing finished streams and re-packing the MB space")
     ap.add_argument("--shape", default=None,
@@ -362,6 +365,7 @@ def main():
     eng = Engine(max_streams=len(specs))
     autotune.apply(eng, tuned)
     eng.config(nat.OPT_CHAIN_LADDER, args.chain_ladder)
+    eng.config(nat.OPT_FUSED_TAILS, args.fused_tails)
     ids = open_streams(eng, specs)
     for _ in range(W - 1):
         eng.step()
@@ -385,6 +389,7 @@ def main():
     eng = Engine(max_streams=len(specs))
     autotune.apply(eng, tuned)
     eng.config(nat.OPT_CHAIN_LADDER, args.chain_ladder)
+    eng.config(nat.OPT_FUSED_TAILS, args.fused_tails)
     eng.config(nat.OPT_OVERLAP_WRITER, 0)
     ids = open_streams(eng, specs)
     for _ in range(W):
@@ -408,6 +413,7 @@ def main():
     eng.config(nat.OPT_DEVICE_BITSTREAM, 0)
     autotune.apply(eng, tuned)
     eng.config(nat.OPT_CHAIN_LADDER, args.chain_ladder)
+    eng.config(nat.OPT_FUSED_TAILS, args.fused_tails)
     ids = open_streams(eng, specs)
     for _ in range(W - 1):
         eng.step()
