@@ -1144,8 +1144,6 @@ int h261g_stream_frames_done(h261g_engine_t *e, int32_t id)
 }
 
 /* per-MB arrays only carry data within a step, so re-packing needs only the
- * map and each descriptor's mb_base rewritten */
-/* per-MB arrays only carry data within a step, so re-packing needs only the
  * map and each descriptor's mb_base rewritten.  The host knows both, so the
  * update is queued (pinned staging copy + a kernel) and nothing waits */
 int h261g_engine_compact(h261g_engine_t *e)
