@@ -415,6 +415,37 @@ def test_put_frames_matches_put_frame(oracle):
         eng.destroy()
 
 
+def test_put_frames_more_streams_than_one_placement_launch():
+    """One h261g_engine_put_frames call for 300 QCIF streams (more than the
+    256 a placement kernel takes, so two launches share the staged copy),
+    with two streams left out of the call, codes what one put_frame per
+    stream codes, over three steps."""
+    ns = 300
+    frames = [seq(900 + k, QCIF, 3, sigma=2, motion=5) for k in range(ns)]
+    a, b = Engine(max_streams=ns), Engine(max_streams=ns)
+    ia = [a.open(*QCIF, fixed_quant=8) for _ in range(ns)]
+    ib = [b.open(*QCIF, fixed_quant=8) for _ in range(ns)]
+    announce(f"put_frames: {ns} QCIF streams in one call")
+    left_out = {7, 263}
+    for t in range(3):
+        fed = [k for k in range(ns) if k not in left_out or t == 0]
+        for k in fed:
+            a.put_frame(ia[k], *frames[k][t])
+        order = fed[::-1]
+        buf, offs = Engine.pack_frames([frames[k][t] for k in order])
+        b.put_frames([ib[k] for k in order], buf, offs)
+        a.step()
+        b.step()
+        a.sync()
+        b.sync()
+        for k in fed:
+            assert b.bitstream(ib[k]) == a.bitstream(ia[k]), f"step {t} stream {k}"
+    assert [b.frames_done(s) for s in ib] == [a.frames_done(s) for s in ia]
+    assert b.frames_done(ib[7]) == 1 and b.frames_done(ib[8]) == 3
+    a.destroy()
+    b.destroy()
+
+
 def test_engine_two_chains_match_one_chain(oracle):
     """Two chains (the MB space split at a stream boundary, each half stepped
     on its own HIP stream), tied by the strict ladder or by the ladder with
