@@ -186,10 +186,11 @@ __device__ __forceinline__ void idct_core(const float *t, float *o)
  * DPP, the four rows by readlane */
 __device__ __forceinline__ int wave_sum_i32(int x)
 {
-    x += __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false);  /* quad_perm [1,0,3,2] */
-    x += __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false);  /* quad_perm [2,3,0,1] */
-    x += __builtin_amdgcn_mov_dpp(x, 0x141, 0xf, 0xf, false); /* row_half_mirror */
-    x += __builtin_amdgcn_mov_dpp(x, 0x140, 0xf, 0xf, false); /* row_mirror */
+    /* "old" = 0, the sum's identity: each move folds into the add */
+    x += __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, false);  /* quad_perm [1,0,3,2] */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, false);  /* quad_perm [2,3,0,1] */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, false); /* row_half_mirror */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, false); /* row_mirror */
     return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) + __builtin_amdgcn_readlane(x, 32) +
            __builtin_amdgcn_readlane(x, 48);
 }
@@ -541,9 +542,9 @@ __global__ __launch_bounds__(64, 6) void mb_kernel(MbLaunch p)
             }
             /* sum the 8 columns of the block: quads, then the two quads of
              * the 8 lanes (row_half_mirror pairs lane j with 7 - j) */
-            blk_accum += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(blk_accum), 0xB1, 0xf, 0xf, false));
-            blk_accum += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(blk_accum), 0x4E, 0xf, 0xf, false));
-            blk_accum += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(blk_accum), 0x141, 0xf, 0xf, false));
+            blk_accum += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(blk_accum), 0xB1, 0xf, 0xf, false));
+            blk_accum += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(blk_accum), 0x4E, 0xf, 0xf, false));
+            blk_accum += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(blk_accum), 0x141, 0xf, 0xf, false));
             const bool coded = L.act && blk_accum > 2.0f;
             /* coded blocks of the pass: ballot of the lanes c == 0 */
             const uint64_t coded_mask = __ballot(coded && rc == 0);

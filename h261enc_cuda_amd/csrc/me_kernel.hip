@@ -73,17 +73,21 @@ constexpr int win_alloc()
  * outside the limits, whose cost is replaced): still inside the row */
 static_assert(tile_dw(1) < 15 && tile_dw(2) < 17 && tile_dw(4) < 29, "tile rows fit their pitch");
 
-/* minimum over the wave: quads, half rows and rows by DPP, rows by readlane */
+/* minimum over the wave: quads, half rows and rows by DPP, rows by readlane.
+ * Every lane reads a lane of its own row, so the DPP "old" operand is never
+ * used; giving it min's identity lets the compiler fold each move into the
+ * v_min_u32 as a DPP source (one instruction per stage instead of a move,
+ * a wait state and the min) */
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 {
     uint32_t o;
-    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, false); /* quad_perm [1,0,3,2] */
+    o = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0xB1, 0xf, 0xf, false); /* quad_perm [1,0,3,2] */
     v = o < v ? o : v;
-    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, false); /* quad_perm [2,3,0,1] */
+    o = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x4E, 0xf, 0xf, false); /* quad_perm [2,3,0,1] */
     v = o < v ? o : v;
-    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false); /* row_half_mirror */
+    o = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x141, 0xf, 0xf, false); /* row_half_mirror */
     v = o < v ? o : v;
-    o = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false); /* row_mirror */
+    o = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x140, 0xf, 0xf, false); /* row_mirror */
     v = o < v ? o : v;
     const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
     const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
@@ -412,10 +416,11 @@ __device__ __forceinline__ int satd_quad(short2v e, short2v o, short2v s1, short
 /* sum over the wave: quads, half rows and rows by DPP, rows by readlane */
 __device__ __forceinline__ int wave_sum(int x)
 {
-    x += __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false);  /* quad_perm [1,0,3,2] */
-    x += __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false);  /* quad_perm [2,3,0,1] */
-    x += __builtin_amdgcn_mov_dpp(x, 0x141, 0xf, 0xf, false); /* row_half_mirror */
-    x += __builtin_amdgcn_mov_dpp(x, 0x140, 0xf, 0xf, false); /* row_mirror */
+    /* "old" = 0, the sum's identity: each move folds into the add */
+    x += __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, false);  /* quad_perm [1,0,3,2] */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, false);  /* quad_perm [2,3,0,1] */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x141, 0xf, 0xf, false); /* row_half_mirror */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x140, 0xf, 0xf, false); /* row_mirror */
     return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) + __builtin_amdgcn_readlane(x, 32) +
            __builtin_amdgcn_readlane(x, 48);
 }
