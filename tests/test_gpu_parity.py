@@ -446,6 +446,77 @@ def test_put_frames_more_streams_than_one_placement_launch():
     b.destroy()
 
 
+def test_engine_api_edges(oracle):
+    """The engine's edges: an engine with nothing open steps, replays and
+    syncs; invalid sizes, quantisers, options, ids and input calls are
+    refused and change nothing; a full engine refuses another stream; a
+    host-fed stream without a new frame and a generated stream past its last
+    frame are not coded; a closed id is refused and its slot is reused."""
+    from h261enc_cuda_amd import _native as nat
+    eng = Engine(max_streams=2)
+    announce("engine API edges")
+    assert eng.step() == 0
+    eng.run(0)
+    eng.run(3)
+    eng.sync()
+    for w, h in ((320, 240), (352, 144), (0, 0), (176, 288)):
+        with pytest.raises(RuntimeError):
+            eng.open(w, h)
+    with pytest.raises(RuntimeError):
+        eng.open(*QCIF, init_quant=0)
+    with pytest.raises(RuntimeError):
+        eng.open(*QCIF, fixed_quant=32)
+    with pytest.raises(RuntimeError):
+        eng.config(999, 0)
+    with pytest.raises(RuntimeError):
+        eng.config(nat.OPT_XCD_REMAP, 8)
+    with pytest.raises(RuntimeError):
+        eng.info(0)  # nothing open
+    a = eng.open(*QCIF, fixed_quant=8)
+    b = eng.open(*CIF, fixed_quant=8)
+    assert (a, b) == (0, 1)
+    with pytest.raises(RuntimeError):
+        eng.open(*QCIF)  # both slots taken
+    with pytest.raises(RuntimeError):
+        eng.generate_input(a, 1, 0)
+    eng.generate_input(b, 5, 2, sigma=2, motion=7)
+    with pytest.raises(RuntimeError):
+        eng.put_frame(b, *zeros(CIF))  # a device-generated stream takes no host frames
+    with pytest.raises(RuntimeError):
+        eng.generate_input(b, 6, 2)  # a stream gets one generated sequence
+    fa = seq(77, QCIF, 1, sigma=2, motion=7)[0]
+    eng.put_frame(a, *fa)
+    eng.step()
+    eng.sync()
+    ob = oracle.stream(*QCIF, fixed_quant=8).encode(*fa)[0]
+    assert eng.bitstream(a) == ob
+    first_a = eng.bitstream(a)
+    # a has no new frame: only b is coded; then b's sequence has ended
+    for _ in range(3):
+        eng.step()
+        eng.sync()
+    assert (eng.frames_done(a), eng.frames_done(b)) == (1, 2)
+    assert eng.bitstream(a) == first_a
+    ob_b = oracle.stream(*CIF, fixed_quant=8)
+    for t in range(2):
+        want_b = ob_b.encode(*content_frame(5, *CIF, t, sigma=2, motion=7))[0]
+    assert eng.bitstream(b) == want_b
+    eng.close(a)
+    with pytest.raises(RuntimeError):
+        eng.close(a)
+    with pytest.raises(RuntimeError):
+        eng.bitstream(a)
+    with pytest.raises(RuntimeError):
+        eng.put_frame(a, *fa)
+    c = eng.open(*QCIF, fixed_quant=8)
+    assert c == a
+    eng.put_frame(c, *fa)
+    eng.step()
+    eng.sync()
+    assert eng.bitstream(c) == ob and eng.frames_done(c) == 1
+    eng.destroy()
+
+
 def test_engine_two_chains_match_one_chain(oracle):
     """Two chains (the MB space split at a stream boundary, each half stepped
     on its own HIP stream), tied by the strict ladder or by the ladder with
