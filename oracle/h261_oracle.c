@@ -223,7 +223,8 @@ static const float k_inv_pre[8] = {1.0, 1.387040, 1.306563, 1.175876, 1.0, 0.785
 
 /* src/mbgpu_kernels.cuh:337-442.  Every product with an unsuffixed literal
  * is evaluated in double and rounded to float on assignment; the final
- * rounding is floorf() of (float)(prod + 0.49999). */
+ * rounding is floorf() of (float)(prod + 0.49999).  Parity unpinned (op
+ * typing; see h261_oracle.h). */
 void ora_fdct8x8(const int32_t in[64], int32_t out[64])
 {
     float t[64];
@@ -285,7 +286,8 @@ void ora_fdct8x8(const int32_t in[64], int32_t out[64])
 }
 
 /* src/mbgpu_kernels.cuh:827-931: prescale, column pass, row pass,
- * (int)floor(double) rounding, clamp to [-256, 255]. */
+ * (int)floor(double) rounding, clamp to [-256, 255].  Parity unpinned (op
+ * typing; see h261_oracle.h). */
 void ora_idct8x8(const int32_t in[64], int32_t out[64])
 {
     float t[64];
