@@ -446,6 +446,38 @@ def test_put_frames_more_streams_than_one_placement_launch():
     b.destroy()
 
 
+def test_put_frames_at_benchmark_scale_matches_generated_input():
+    """512 host-fed streams (256 CIF + 256 QCIF: two chains, full-size
+    launches) fed by one h261g_engine_put_frames call per step code exactly
+    what the same streams code from device-generated input, every stream's
+    bitstream compared at every step."""
+    ns, steps = 512, 3
+    sizes = [CIF] * 256 + [QCIF] * 256
+    a, b = Engine(max_streams=ns), Engine(max_streams=ns)
+    ia, ib = [], []
+    for k, size in enumerate(sizes):
+        rate = 64000 + 1000 * (k % 320)
+        ia.append(a.open(*size, bitrate=rate, vbv_size=rate))
+        ib.append(b.open(*size, bitrate=rate, vbv_size=rate))
+        a.generate_input(ia[-1], 700 + k, steps, sigma=2, motion=15)
+    announce(f"put_frames at scale: {ns} streams, {steps} steps")
+    for t in range(steps):
+        buf, offs = Engine.pack_frames([content_frame(700 + k, *sizes[k], t, sigma=2, motion=15)
+                                        for k in range(ns)])
+        b.put_frames(ib, buf, offs)
+        a.step()
+        b.step()
+        a.fetch_bitstreams(ia)
+        got_a = a.fetched()
+        b.fetch_bitstreams(ib)
+        got_b = b.fetched()
+        for k in range(ns):
+            assert got_b[k] == got_a[k], f"step {t} stream {k}"
+    assert [b.frames_done(s) for s in ib] == [steps] * ns
+    a.destroy()
+    b.destroy()
+
+
 def test_engine_api_edges(oracle):
     """The engine's edges: an engine with nothing open steps, replays and
     syncs; invalid sizes, quantisers, options, ids and input calls are
