@@ -19,8 +19,12 @@ using content::FrameCtx;
 
 namespace {
 
-/* blockIdx.y = frame of the job; threads cover 4 luma pels or 4 chroma
- * pel pairs each */
+/* blockIdx.y = frame of the job.  A thread takes a 4x2 block of luma pels
+ * and the two chroma pels under it.  The block lies inside one macroblock,
+ * so its displacement is computed once, and a chroma pel's texture sum is
+ * the sum of the noise-free luma textures of its 2x2 pels: 8 texture
+ * evaluations per thread serve both planes (content::luma and
+ * content::chroma per pel would take 16, and a displacement per pel). */
 __global__ __launch_bounds__(256) void content_kernel(const FrameCtx *ctx, uint8_t *out, int width, int height,
                                                       size_t frame_stride)
 {
@@ -28,26 +32,35 @@ __global__ __launch_bounds__(256) void content_kernel(const FrameCtx *ctx, uint8
     uint8_t *y = out + frame_stride * blockIdx.y;
     uint8_t *cb = y + (size_t)width * height;
     uint8_t *cr = cb + (size_t)width * height / 4;
-    const int n_luma = width * height / 4, n_chroma = width * height / 16;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_luma) {
-        const int px = (4 * i) % width, py = (4 * i) / width;
+    const int bw = width / 4, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bw * (height / 2)) return;
+    const int x0 = 4 * (i % bw), y0 = 2 * (i / bw);
+    int dx, dy;
+    content::displacement(f, x0, y0, &dx, &dy);
+    int t[2][4];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) t[r][k] = content::texture(f.tex_seed, x0 + k + dx, y0 + r + dy);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
         uint32_t w = 0;
-        for (int k = 0; k < 4; k++) w |= (uint32_t)content::luma(f, px + k, py) << (8 * k);
-        *(uint32_t *)(y + 4 * (size_t)i) = w;
-    } else if (i < n_luma + n_chroma) {
-        const int j = i - n_luma, cw = width / 2;
-        const int px = (4 * j) % cw, py = (4 * j) / cw;
-        uint32_t wb = 0, wr = 0;
-        for (int k = 0; k < 4; k++) {
-            uint8_t b, r;
-            content::chroma(f, px + k, py, &b, &r);
-            wb |= (uint32_t)b << (8 * k);
-            wr |= (uint32_t)r << (8 * k);
-        }
-        *(uint32_t *)(cb + 4 * (size_t)j) = wb;
-        *(uint32_t *)(cr + 4 * (size_t)j) = wr;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            w |= (uint32_t)content::clamp255(t[r][k] + content::noise(f.noise_seed, f.frame, x0 + k, y0 + r, f.sigma))
+                 << (8 * k);
+        *(uint32_t *)(y + (size_t)(y0 + r) * width + x0) = w;
     }
+    uint32_t wb = 0, wr = 0;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const int sum = t[0][2 * c] + t[0][2 * c + 1] + t[1][2 * c] + t[1][2 * c + 1];
+        wb |= (uint32_t)content::clamp255(128 + (sum - 504) / 4) << (8 * c);
+        wr |= (uint32_t)content::clamp255(128 - (sum - 504) / 8) << (8 * c);
+    }
+    const size_t co = (size_t)(y0 / 2) * (width / 2) + x0 / 2;
+    *(uint16_t *)(cb + co) = (uint16_t)wb;
+    *(uint16_t *)(cr + co) = (uint16_t)wr;
 }
 
 /* FrameCtx of frames first .. first + n - 1, one workgroup: the same
@@ -118,7 +131,7 @@ int generate_frames_device(const h261g_content_params_t &c, int width, int heigh
     hipLaunchKernelGGL(content_ctx_kernel, dim3(1), dim3(kCtxThreads), 0, (hipStream_t)stream, c, width, height, first,
                        n, ctx);
     if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
-    const int threads = width * height / 4 + width * height / 16;
+    const int threads = (width / 4) * (height / 2);
     hipLaunchKernelGGL(content_kernel, dim3((threads + 255) / 256, n), dim3(256), 0, (hipStream_t)stream, ctx, out,
                        width, height, (size_t)width * height * 3 / 2);
     return hipGetLastError() == hipSuccess ? 0 : H261G_ERR_DEVICE;

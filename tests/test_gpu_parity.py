@@ -478,6 +478,30 @@ def test_put_frames_at_benchmark_scale_matches_generated_input():
     b.destroy()
 
 
+@pytest.mark.parametrize("local", [False, True], ids=["global-motion", "local-motion-scene-cuts"])
+def test_generated_input_equals_host_generator(local):
+    """Frames that h261g_stream_generate_input writes on the device equal the
+    host generator's (content_frame) plane for plane, frame after frame: CIF
+    and QCIF, global motion, or local motion across scene cuts."""
+    kw = dict(sigma=4, motion=15, local=True, scene_cut=3) if local else dict(sigma=2, motion=15)
+    specs = [(CIF, 931), (QCIF, 932)]
+    n = 7
+    eng = Engine(max_streams=len(specs))
+    sids = [eng.open(*size, fixed_quant=8) for size, _ in specs]
+    for sid, (size, seed) in zip(sids, specs):
+        eng.generate_input(sid, seed, n, **kw)
+    announce(f"generated input vs host generator, local={local}")
+    for t in range(n):
+        for sid, (size, seed) in zip(sids, specs):
+            got = eng.input(sid)
+            want = content_frame(seed, *size, t, **kw)
+            for p in range(3):
+                assert np.array_equal(got[p], want[p]), f"frame {t} {size} plane {p}"
+        eng.step()
+        eng.sync()
+    eng.destroy()
+
+
 def test_engine_api_edges(oracle):
     """The engine's edges: an engine with nothing open steps, replays and
     syncs; invalid sizes, quantisers, options, ids and input calls are
