@@ -127,6 +127,11 @@ H261G_CHD void local_vector(uint64_t sseed, int i, int j, int width, int height,
     *vy = dy < -15 ? -15 : (dy > 15 ? 15 : dy);
 }
 
+/* the noise-free texture of one seed over the whole 512x512 torus, one byte
+ * per pel (values 16..235): the device generator fills it once per scene
+ * and every frame of the scene reads its pels from it */
+constexpr int kTextureTableBytes = 512 * 512;
+
 struct FrameCtx {
     uint64_t tex_seed; /* texture of this frame (scene seed in local mode) */
     uint64_t noise_seed;

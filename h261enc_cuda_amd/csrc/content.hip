@@ -1,8 +1,9 @@
 /*
  * content.hip -- synthetic input frames (content.h) on the device.  The
- * kernel writes whole frame sequences straight into HBM so that encoding
- * throughput is measured with inputs already resident.  The same frames on
- * the host: content_host.cpp.
+ * kernels write whole frame sequences straight into HBM so that encoding
+ * throughput is measured with inputs already resident: per scene, the
+ * texture once into a 512x512 table, then the scene's frames from it.  The
+ * same frames on the host: content_host.cpp.
  */
 #include <hip/hip_runtime.h>
 
@@ -19,14 +20,22 @@ using content::FrameCtx;
 
 namespace {
 
-/* blockIdx.y = frame of the job.  A thread takes a 4x2 block of luma pels
- * and the two chroma pels under it.  The block lies inside one macroblock,
- * so its displacement is computed once, and a chroma pel's texture sum is
- * the sum of the noise-free luma textures of its 2x2 pels: 8 texture
- * evaluations per thread serve both planes (content::luma and
- * content::chroma per pel would take 16, and a displacement per pel). */
-__global__ __launch_bounds__(256) void content_kernel(const FrameCtx *ctx, uint8_t *out, int width, int height,
-                                                      size_t frame_stride)
+/* the texture of one seed over the 512x512 torus, a byte per pel
+ * (content::texture, 16..235) */
+__global__ __launch_bounds__(256) void texture_table_kernel(uint64_t seed, uint8_t *tab)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < content::kTextureTableBytes) tab[i] = (uint8_t)content::texture(seed, i & 511, i >> 9);
+}
+
+/* blockIdx.y = frame of the job (frames of one scene: their texture is
+ * tab).  A thread takes a 4x2 block of luma pels and the two chroma pels
+ * under it.  The block lies inside one macroblock, so its displacement is
+ * computed once, and a chroma pel's texture sum is the sum of the
+ * noise-free luma textures of its 2x2 pels: 8 table reads per thread serve
+ * both planes, and only the per-pel noise is hashed. */
+__global__ __launch_bounds__(256) void content_kernel(const FrameCtx *ctx, const uint8_t *tab, uint8_t *out, int width,
+                                                      int height, size_t frame_stride)
 {
     const FrameCtx f = ctx[blockIdx.y];
     uint8_t *y = out + frame_stride * blockIdx.y;
@@ -41,7 +50,7 @@ __global__ __launch_bounds__(256) void content_kernel(const FrameCtx *ctx, uint8
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) t[r][k] = content::texture(f.tex_seed, x0 + k + dx, y0 + r + dy);
+        for (int k = 0; k < 4; k++) t[r][k] = tab[((y0 + r + dy) & 511) * 512 + ((x0 + k + dx) & 511)];
 #pragma unroll
     for (int r = 0; r < 2; r++) {
         uint32_t w = 0;
@@ -121,20 +130,36 @@ __global__ __launch_bounds__(kCtxThreads) void content_ctx_kernel(h261g_content_
 
 }  // namespace
 
-/* frames [first, first + n) of a stream into out (I420, frame after frame) */
+/* frames [first, first + n) of a stream into out (I420, frame after frame);
+ * tab: content::kTextureTableBytes of device memory, used on this stream only */
 int generate_frames_device(const h261g_content_params_t &c, int width, int height, int first, int n, uint8_t *out,
-                           FrameCtx *ctx, void *stream)
+                           FrameCtx *ctx, uint8_t *tab, void *stream)
 {
     if (n <= 0) return 0;
-    /* per-frame parameters on the device, then the frames: two launches,
-     * nothing for the host to wait for */
+    /* per-frame parameters on the device, then scene by scene the texture
+     * table and the scene's frames: nothing for the host to wait for */
     hipLaunchKernelGGL(content_ctx_kernel, dim3(1), dim3(kCtxThreads), 0, (hipStream_t)stream, c, width, height, first,
                        n, ctx);
     if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
     const int threads = (width / 4) * (height / 2);
-    hipLaunchKernelGGL(content_kernel, dim3((threads + 255) / 256, n), dim3(256), 0, (hipStream_t)stream, ctx, out,
-                       width, height, (size_t)width * height * 3 / 2);
-    return hipGetLastError() == hipSuccess ? 0 : H261G_ERR_DEVICE;
+    const size_t fsz = (size_t)width * height * 3 / 2;
+    for (int a = first; a < first + n;) {
+        /* the texture seed of frame a's scene, as content_ctx_kernel sets it */
+        int b = first + n;
+        uint64_t seed = c.seed;
+        if (c.local) {
+            const int scene = c.scene_cut > 0 ? a / c.scene_cut : 0;
+            if (c.scene_cut > 0) b = min(b, (scene + 1) * c.scene_cut);
+            seed = content::scene_seed(c.seed, scene);
+        }
+        hipLaunchKernelGGL(texture_table_kernel, dim3(content::kTextureTableBytes / 256), dim3(256), 0,
+                           (hipStream_t)stream, seed, tab);
+        hipLaunchKernelGGL(content_kernel, dim3((threads + 255) / 256, b - a), dim3(256), 0, (hipStream_t)stream,
+                           ctx + (a - first), tab, out + (size_t)(a - first) * fsz, width, height, fsz);
+        if (hipGetLastError() != hipSuccess) return H261G_ERR_DEVICE;
+        a = b;
+    }
+    return 0;
 }
 
 }  // namespace h261g

@@ -40,7 +40,7 @@
 namespace h261g {
 
 int generate_frames_device(const h261g_content_params_t &c, int width, int height, int first, int n, uint8_t *out,
-                           content::FrameCtx *ctx, void *stream);
+                           content::FrameCtx *ctx, uint8_t *tab, void *stream);
 
 namespace {
 
@@ -1205,9 +1205,12 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
     const int64_t fsz = (int64_t)s->width * s->height * 3 / 2;
     const int64_t bytes = fsz * n_frames;
     if (e->budget && e->used + bytes > e->budget) return H261G_ERR_FULL;
-    /* the frames, then the generator's per-frame parameters */
+    /* the frames, then the generator's per-frame parameters and its texture
+     * table (one per stream, so generations on different HIP streams never
+     * share one) */
     const size_t ctx_off = ((size_t)bytes + 255) & ~(size_t)255;
-    s->seq = cache_get(e, ctx_off + sizeof(content::FrameCtx) * n_frames, &s->seq_cap);
+    const size_t tab_off = (ctx_off + sizeof(content::FrameCtx) * n_frames + 255) & ~(size_t)255;
+    s->seq = cache_get(e, tab_off + content::kTextureTableBytes, &s->seq_cap);
     if (!s->seq) return H261G_ERR_MEMORY;
     /* asynchronous input (a fresh stream only): generated on the input
      * stream after the compute work queued so far (a cached buffer may still
@@ -1219,7 +1222,8 @@ int h261g_stream_generate_input(h261g_engine_t *e, int32_t id, const h261g_conte
                   !ok(hipStreamWaitEvent(e->in_stream, e->compute_ev, 0), "wait")))
         return H261G_ERR_DEVICE;
     int rcode = generate_frames_device(*c, s->width, s->height, 0, n_frames, s->seq,
-                                       (content::FrameCtx *)(s->seq + ctx_off), async ? e->in_stream : e->stream);
+                                       (content::FrameCtx *)(s->seq + ctx_off), s->seq + tab_off,
+                                       async ? e->in_stream : e->stream);
     if (rcode) {
         cache_put(e, s->seq, s->seq_cap);
         s->seq = nullptr;
