@@ -98,33 +98,44 @@ class ServingLoop:
     def run(self, requests: List[Request], collect: bool = False, realtime: bool = True, deliver: bool = False):
         """Serves the requests.  deliver=True brings every step's bitstreams
         to host memory before the step counts as done; collect=True (implies
-        deliver) also returns every request's per-frame bitstreams."""
+        deliver) also returns every request's per-frame bitstreams.
+
+        With deliver, one step runs ahead: step k+1 is queued before step k's
+        bitstreams are collected, so the host's bookkeeping (admissions,
+        latencies, closes) overlaps the device's next step instead of leaving
+        it idle.  Which streams code a frame in a step is known when it is
+        queued (input ready, frames left), so the per-request state is
+        updated then; a request is closed once its last frame's bitstream
+        has arrived."""
         deliver = deliver or collect
         eng = self.eng
-        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        R = len(requests)
+        pending = collections.deque(sorted(range(R), key=lambda i: requests[i].arrival))
         waiting: collections.deque = collections.deque()
-        index = {id(r): i for i, r in enumerate(requests)}
         outputs: Dict[int, List[bytes]] = {}
-        # active streams as parallel arrays (per-step bookkeeping is vectorised)
-        sids = np.zeros(0, np.int64)
-        req_idx = np.zeros(0, np.int64)
-        eligible = np.zeros(0)
-        arrival = np.zeros(0)
-        left = np.zeros(0, np.int64)
-        nmb = np.zeros(0, np.int64)
-        started = np.zeros(0, bool)  # input ready: the stream codes a frame per step
+        # per-request state, indexed by the request's position in `requests`
+        sid_of = np.full(R, -1, np.int64)
+        eligible = np.zeros(R)
+        arrival = np.array([r.arrival if realtime else 0.0 for r in requests]) if R else np.zeros(0)
+        left = np.array([r.frames for r in requests], np.int64) if R else np.zeros(0, np.int64)
+        nmb = np.array([(r.width // 16) * (r.height // 16) for r in requests], np.int64) if R else np.zeros(0, np.int64)
+        started = np.zeros(R, bool)  # input ready: the stream codes a frame per step
+        active = np.zeros(R, bool)   # open on the engine
         frame_lat: List[np.ndarray] = []
         req_lat: List[float] = []
         input_wait: List[float] = []
         t_start = time.perf_counter()
         frames = mbs = steps = max_conc = delivered = 0
-        while pending or waiting or len(sids):
+        # the queued step whose results are not collected yet: the requests
+        # that code a frame in it, and those for which it is the last frame
+        inflight = None
+        while pending or waiting or active.any() or inflight is not None:
             now = time.perf_counter() - t_start
-            while pending and (pending[0].arrival <= now or not realtime):
+            while pending and (requests[pending[0]].arrival <= now or not realtime):
                 waiting.append(pending.popleft())
-            new = []
             while waiting:
-                r = waiting[0]
+                ri = waiting[0]
+                r = requests[ri]
                 try:
                     sid = eng.open(r.width, r.height, denoise=5, init_quant=10, bitrate=r.bitrate,
                                    vbv_size=r.bitrate)
@@ -137,57 +148,56 @@ class ServingLoop:
                     eng.close(sid)
                     break
                 waiting.popleft()
-                new.append((sid, r))
+                sid_of[ri] = sid
+                active[ri] = True
                 if collect:
-                    outputs[index[id(r)]] = []
-            if new:
-                t_adm = time.perf_counter() - t_start
-                sids = np.concatenate([sids, [sid for sid, _ in new]])
-                req_idx = np.concatenate([req_idx, [index[id(r)] for _, r in new]])
-                eligible = np.concatenate([eligible, np.full(len(new), t_adm)])
-                arrival = np.concatenate([arrival, [r.arrival if realtime else 0.0 for _, r in new]])
-                left = np.concatenate([left, [r.frames for _, r in new]])
-                nmb = np.concatenate([nmb, [(r.width // 16) * (r.height // 16) for _, r in new]])
-                started = np.concatenate([started, np.zeros(len(new), bool)])
-            if not len(sids):
-                if pending and realtime:
-                    time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t_start), 0.01)))
-                continue
-            max_conc = max(max_conc, len(sids))
-            t_call = time.perf_counter() - t_start
-            eng.step()
-            # streams activated by this step (their input became ready) code
-            # their first frame in it: eligible from this step's start
-            for k in np.nonzero(~started)[0].tolist():
-                if not eng.input_pending(int(sids[k])):
-                    started[k] = True
-                    eligible[k] = t_call
-                    input_wait.append((t_call - arrival[k]) * 1e3)
-            run = started
-            if deliver:
-                eng.fetch_bitstreams(sids[run])
-            eng.sync()
-            if deliver:
+                    outputs[ri] = []
+            act = np.nonzero(active)[0]
+            queued = None
+            if len(act):
+                max_conc = max(max_conc, len(act))
+                t_call = time.perf_counter() - t_start
+                eng.step()
+                steps += 1
+                # streams activated by this step (their input became ready)
+                # code their first frame in it: eligible from this step's start
+                for ri in act[~started[act]].tolist():
+                    if not eng.input_pending(int(sid_of[ri])):
+                        started[ri] = True
+                        eligible[ri] = t_call
+                        input_wait.append((t_call - arrival[ri]) * 1e3)
+                run = act[started[act] & (left[act] > 0)]
+                if len(run):
+                    left[run] -= 1
+                    queued = (run, run[left[run] == 0])
+            if deliver and inflight is not None:
+                # step k's bitstreams (the fetch waits for step k only; step
+                # k+1, queued above, keeps the device busy meanwhile)
+                ran, last = inflight
                 if collect:
-                    for ri, bits in zip(req_idx[run].tolist(), eng.fetched()):
+                    for ri, bits in zip(ran.tolist(), eng.fetched()):
                         outputs[ri].append(bits)
                 else:
                     delivered += eng.fetch_wait()
-            steps += 1
-            t_end = time.perf_counter() - t_start
-            frame_lat.append((t_end - eligible[run]) * 1e3)
-            eligible[run] = t_end
-            left[run] -= 1
-            frames += int(run.sum())
-            mbs += int(nmb[run].sum())
-            done = left <= 0
-            if done.any():
-                for sid in sids[done].tolist():
-                    eng.close(sid)
-                req_lat.extend(((t_end - arrival[done]) * 1e3).tolist())
-                keep = ~done
-                sids, req_idx, eligible, arrival, left, nmb, started = (
-                    sids[keep], req_idx[keep], eligible[keep], arrival[keep], left[keep], nmb[keep], started[keep])
+                self._step_done(ran, last, eng, sid_of, eligible, arrival, active, frame_lat, req_lat, t_start)
+                frames += len(ran)
+                mbs += int(nmb[ran].sum())
+                inflight = None
+            if queued is not None:
+                if deliver:
+                    eng.fetch_bitstreams(sid_of[queued[0]])
+                    inflight = queued
+                else:
+                    eng.sync()
+                    self._step_done(*queued, eng, sid_of, eligible, arrival, active, frame_lat, req_lat, t_start)
+                    frames += len(queued[0])
+                    mbs += int(nmb[queued[0]].sum())
+            elif len(act):
+                # a step that coded nothing (inputs still being generated):
+                # wait for it rather than queue empty steps
+                eng.sync()
+            elif inflight is None and pending and realtime:
+                time.sleep(max(0.0, min(requests[pending[0]].arrival - (time.perf_counter() - t_start), 0.01)))
         seconds = time.perf_counter() - t_start
         fl = np.concatenate(frame_lat) if frame_lat else np.zeros(1)
         rl = np.array(req_lat) if req_lat else np.zeros(1)
@@ -201,6 +211,19 @@ class ServingLoop:
                              input_wait_ms_p99=float(np.percentile(iw, 99)), max_concurrent=max_conc,
                              steps=steps, delivered_bytes=delivered)
         return (stats, outputs) if collect else stats
+
+    @staticmethod
+    def _step_done(ran, last, eng, sid_of, eligible, arrival, active, frame_lat, req_lat, t_start):
+        """The frames of a step are done (on the device, or delivered): their
+        latencies, each stream's next frame eligible from now, and the
+        requests whose last frame this was (`last`) closed."""
+        t_end = time.perf_counter() - t_start
+        frame_lat.append((t_end - eligible[ran]) * 1e3)
+        eligible[ran] = t_end
+        for ri in last.tolist():
+            eng.close(int(sid_of[ri]))
+            active[ri] = False
+            req_lat.append((t_end - arrival[ri]) * 1e3)
 
     def close(self):
         self.eng.destroy()
