@@ -12,7 +12,9 @@ available and its previous frame done) to the end of the step that coded it
 (levels, side information, reconstruction and the H.261 bitstream on the
 device) -- with `deliver`, to the bitstream's arrival in host memory (every
 step's bitstreams are gathered into pinned host memory by one queued
-operation).  A request's input is a synthetic sequence generated on the
+operation, and the loop queues the next step before it collects them, so
+the device never waits for the host's bookkeeping).  A request's input is a
+synthetic sequence generated on the
 device; the time from its arrival until that input is available is reported
 separately as the input wait, and is part of the request latency (arrival to
 last frame).
